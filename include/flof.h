@@ -1,0 +1,293 @@
+/*
+ * flof.h -- C-ABI of the MI355X FlOF 4D deformation library (libflof_hip.so).
+ *
+ * The entry points replace the mantaflow plugin surface that scenes/flof.py
+ * and the 4D tests call.  Each declaration cites the reference function it
+ * stands in for (paths relative to the ofblend source tree).  Plain pointers,
+ * sizes and PODs only: no C++ or torch types cross this boundary, no exception
+ * escapes it.  Every function returns FLOF_OK (0) or a negative status; the
+ * message of the last failure is available from flof_last_error().
+ *
+ * Threading: one flof_ctx per host thread.  All work is enqueued on the
+ * context's HIP stream.  Functions that return a host value (reductions,
+ * solve statistics, downloads) synchronise the stream before returning, the
+ * others return after enqueueing (call flof_sync() to wait), mirroring the
+ * synchronous plugin calls of the reference when the shim syncs.
+ */
+#ifndef FLOF_H
+#define FLOF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum flof_status {
+    FLOF_OK = 0,
+    FLOF_ERR_ARG = -1,       /* bad argument / null pointer                  */
+    FLOF_ERR_DIM = -2,       /* grid dimensions mismatch or too small        */
+    FLOF_ERR_TYPE = -3,      /* element type not supported by the function   */
+    FLOF_ERR_HIP = -4,       /* HIP runtime error                            */
+    FLOF_ERR_NOMEM = -5,     /* device allocation failed                     */
+    FLOF_ERR_STATE = -6,     /* solver state invalid (e.g. dt != 1)          */
+    FLOF_ERR_RANGE = -7      /* value outside the supported range            */
+};
+
+/* Element types; values equal Grid4dBase::Grid4dType (source/grid4d.h:29-35). */
+enum flof_elem {
+    FLOF_REAL = 1,
+    FLOF_INT = 2,
+    FLOF_VEC4 = 8
+};
+
+/* Storage precision of a grid.  F32 is the reference's `Real`
+ * (source/util/vectorbase.h:42-51); BF16/F16 are the mixed-precision storage
+ * variants (arithmetic always accumulates in FP32, CG scalars in FP64). */
+enum flof_dtype {
+    FLOF_F32 = 0,
+    FLOF_BF16 = 1,
+    FLOF_F16 = 2
+};
+
+typedef struct flof_ctx flof_ctx;
+typedef struct flof_grid4d flof_grid4d;
+
+/* Parameters of opticalFlowMultiscale4d / opticalFlow4d.  Field names and
+ * defaults follow the keyword arguments of
+ * source/plugin/optflow4d.cpp:2182-2189 (multiscale) and :2110-2114 (single
+ * level).  Fill with flof_of_params_default(). */
+typedef struct flof_of_params {
+    double wSmooth;        /* 0.     smoothness weight                        */
+    double wEnergy;        /* 0.     Tikhonov weight                          */
+    int32_t level;         /* 0      recursion level of the call              */
+    double postVelBlur;    /* 0.     Gaussian blur of each solve (sigma*2)    */
+    double cgAccuracy;     /* 1e-4   relative signed-max residual target      */
+    int32_t blurType;      /* 1      only 1 (Gaussian) is defined             */
+    double cfl;            /* 999    advection CFL                            */
+    int32_t orderTime;     /* 1      unused by the reference                  */
+    int32_t orderSpace;    /* 1      unused by the reference                  */
+    double resetBndWidth;  /* -1.    zero band after each solve (fraction)    */
+    int32_t multiStep;     /* 1      OF solves per level                      */
+    int32_t projSizeThresh;/* 9999   projection-only solve above this size    */
+    int32_t minGridSize;   /* 10     recursion stops at X <= minGridSize      */
+    int32_t doFinalProject;/* 0      level-set projection at level 0          */
+    int32_t cgMaxIter;     /* 1000   CG iteration cap (optflow4d.cpp:502)     */
+    /* Extensions of this library (no reference counterpart): */
+    int32_t warmStart;     /* 0      1: use *warm as CG start at each level   */
+    int32_t pad_;
+} flof_of_params;
+
+#define FLOF_MAX_LEVELS 16
+#define FLOF_MAX_STEPS 10
+
+/* Convergence record of one call; level 0 is the finest level. */
+typedef struct flof_solve_info {
+    int32_t levels;                              /* pyramid levels used      */
+    int32_t dims[FLOF_MAX_LEVELS][4];            /* X,Y,Z,T per level        */
+    int32_t steps[FLOF_MAX_LEVELS];              /* OF solves run per level  */
+    int32_t iters[FLOF_MAX_LEVELS][FLOF_MAX_STEPS];  /* CG iterations        */
+    double residual[FLOF_MAX_LEVELS][FLOF_MAX_STEPS];/* final signed-max
+                                                    residual / initial one  */
+    int32_t status[FLOF_MAX_LEVELS][FLOF_MAX_STEPS]; /* 0 converged, 1 not
+                                                    converged, 2 NaN/zero
+                                                    sigma, 3 rhs below eps  */
+    double err_level[FLOF_MAX_LEVELS];           /* calcLsDiff after the
+                                                    level's pre-warp         */
+    double err_pre;                              /* error before any solve   */
+    double err_final;                            /* error after final warp   */
+    int32_t warm_used[FLOF_MAX_LEVELS];          /* 1: warm start kept, 0:
+                                                    cold start (or fallback) */
+    int32_t pad_;
+    int64_t total_iters;
+} flof_solve_info;
+
+/* ---- context ----------------------------------------------------------- */
+void flof_of_params_default(flof_of_params *p);
+int flof_ctx_create(flof_ctx **out, int device);
+int flof_ctx_destroy(flof_ctx *ctx);
+const char *flof_last_error(const flof_ctx *ctx);
+int flof_sync(flof_ctx *ctx);
+/* HIP stream (hipStream_t) the context enqueues on. */
+void *flof_ctx_stream(flof_ctx *ctx);
+/* Release cached temporaries of the context's device pool. */
+int flof_ctx_trim(flof_ctx *ctx);
+/* Per-kernel timing with HIP events on the context stream.  While enabled,
+ * every launch of kernel class `kind` (FLOF_KT_*) is bracketed by events;
+ * flof_ctx_timing returns the number of launches and their summed duration. */
+enum flof_kernel_kind {
+    FLOF_KT_CG_APPLY = 0,   /* q = A p and p.q                         */
+    FLOF_KT_CG_UPDATE = 1,  /* x,r update, signed max, z.r             */
+    FLOF_KT_CG_DIR = 2,     /* p = z + beta p                          */
+    FLOF_KT_WARP = 3,       /* 3D frame warp                           */
+    FLOF_KT_ADVECT = 4,     /* 4D semi-Lagrangian advection            */
+    FLOF_KT_COUNT = 8
+};
+int flof_ctx_enable_timing(flof_ctx *ctx, int kind, int enable);
+int flof_ctx_timing(flof_ctx *ctx, int kind, int64_t *launches, double *ms_total);
+
+/* ---- Grid4d storage (source/grid4d.h:117-265, source/grid4d.cpp:60-138) - */
+/* Creates a zero-initialised X*Y*Z*T grid (x fastest, then y, z, t). */
+int flof_grid4d_create(flof_ctx *ctx, int X, int Y, int Z, int T, int elem,
+                       int dtype, flof_grid4d **out);
+int flof_grid4d_destroy(flof_grid4d *g);
+int flof_grid4d_dims(const flof_grid4d *g, int dims[4], int *elem, int *dtype);
+/* Device address of the grid's storage (for zero-copy interop). */
+void *flof_grid4d_device_ptr(flof_grid4d *g);
+/* Host <-> device copies of the whole grid; `bytes` must equal its size. */
+int flof_grid4d_upload(flof_grid4d *g, const void *host, size_t bytes);
+int flof_grid4d_download(const flof_grid4d *g, void *host, size_t bytes);
+/* Grid4d::clear / copyFrom / swap (grid4d.cpp:99-122,198-207). */
+int flof_grid4d_clear(flof_grid4d *g);
+int flof_grid4d_copy_from(flof_grid4d *dst, const flof_grid4d *src);
+int flof_grid4d_swap(flof_grid4d *a, flof_grid4d *b);
+/* Elementwise ops (grid4d.cpp:208-245).  `value` points to 1 float for REAL
+ * and INT (converted) grids, 4 floats for VEC4 grids. */
+int flof_grid4d_set_const(flof_grid4d *g, const float *value);
+int flof_grid4d_add_const(flof_grid4d *g, const float *value);
+int flof_grid4d_mult_const(flof_grid4d *g, const float *value);
+int flof_grid4d_add(flof_grid4d *g, const flof_grid4d *a);
+int flof_grid4d_sub(flof_grid4d *g, const flof_grid4d *a);
+int flof_grid4d_mult(flof_grid4d *g, const flof_grid4d *a);
+int flof_grid4d_add_scaled(flof_grid4d *g, const flof_grid4d *a,
+                           const float *factor);
+int flof_grid4d_clamp(flof_grid4d *g, float vmin, float vmax);
+/* getMax / getMin / getMaxAbs (grid4d.cpp:247-270): for VEC4 grids the
+ * square root of the max/min squared norm. */
+int flof_grid4d_get_max(flof_grid4d *g, float *out);
+int flof_grid4d_get_min(flof_grid4d *g, float *out);
+int flof_grid4d_get_max_abs(flof_grid4d *g, float *out);
+/* setBound / setBoundNeumann (grid4d.cpp:355-412). */
+int flof_grid4d_set_bound(flof_grid4d *g, const float *value, int width);
+int flof_grid4d_set_bound_neumann(flof_grid4d *g, int width);
+/* grid4dMaxDiff / grid4dMaxDiffVec4 (grid4d.cpp:419-464). */
+int flof_grid4d_max_diff(const flof_grid4d *a, const flof_grid4d *b,
+                         double *out);
+/* setRegion4d / setRegion4dVec4 (grid4d.cpp:466-483). */
+int flof_set_region4d(flof_grid4d *dst, const float start[4],
+                      const float end[4], const float *value);
+/* getSliceFrom4d (grid4d.cpp:486-499): dst is a host-visible-size 3D device
+ * buffer of dx*dy*dz floats. */
+int flof_get_slice_from4d(flof_grid4d *src, int srct, float *dst3d_dev,
+                          int dx, int dy, int dz);
+/* placeGrid3d (grid4d.cpp:516-524). */
+int flof_place_grid3d(const float *src3d_dev, int sx, int sy, int sz,
+                      flof_grid4d *dst, int dstt);
+
+/* ---- resampling (grid4d.cpp:531-569, optflow4d.cpp:40-57) -------------- */
+/* interpolateGrid4d / interpolateGrid4dVec: target(c) = interp(source,
+ * c*f + off) with f,off from gridFactor4d(offset, scale, size). */
+int flof_interpolate_grid4d(flof_grid4d *target, flof_grid4d *source,
+                            const float offset[4], const float scale[4],
+                            const float size[4]);
+
+/* ---- advection (optflow4d.cpp:1275-1321,2170-2180) --------------------- */
+/* advect4d(vel, grid, dtFac) with solver timestep `dt`: semi-Lagrangian
+ * quadrilinear backtrace of `grid` (REAL or VEC4) by dt*dtFac*vel. */
+int flof_advect4d(flof_grid4d *vel, flof_grid4d *grid, float dt, float dtFac);
+/* advectCflTemplate<Grid4d>: CFL-substepped advection by velFactor*vel. */
+int flof_advect_cfl4d(flof_grid4d *vel, flof_grid4d *grid, float cfl,
+                      float velFactor, float dt);
+
+/* ---- error metric (optflow4d.cpp:895-927,2132-2137) -------------------- */
+int flof_calc_ls_diff4d(flof_grid4d *i0, flof_grid4d *i1, flof_grid4d *out,
+                        float correction, int bnd, float *result);
+
+/* ---- blur (optflow4d.cpp:127-174) -------------------------------------- */
+int flof_gaussian_blur4d(flof_grid4d *grid, float sigma, int iter);
+
+/* ---- optical flow (optflow4d.cpp:361-553,2110-2119) -------------------- */
+/* opticalFlow4d: one level.  vel (VEC4) holds the incoming flow used by the
+ * right-hand side and receives the solution.  rhsT (REAL, may be NULL)
+ * receives component 0 of the right-hand side. */
+int flof_optical_flow4d(flof_grid4d *vel, flof_grid4d *i0, flof_grid4d *i1,
+                        flof_grid4d *rhsT, const flof_of_params *p,
+                        flof_solve_info *info);
+/* opticalFlowMultiscale4d (optflow4d.cpp:936-1173,2182-2195).  Returns the
+ * level-0 error through *err (the reference's return value).  When
+ * p->warmStart is set, `warm` (VEC4, finest size) is an initial guess of the
+ * finest-level CG (restricted for coarser levels); the converged flow is the
+ * one of the same linear systems. */
+int flof_optical_flow_multiscale4d(flof_grid4d *vel, flof_grid4d *i0,
+                                   flof_grid4d *i1, flof_grid4d *rhsT,
+                                   const flof_of_params *p,
+                                   flof_grid4d *warm, flof_solve_info *info,
+                                   float *err);
+/* corrVelsOf4d (optflow4d.cpp:651-802,2121-2130). */
+int flof_corr_vels_of4d(flof_grid4d *dst, flof_grid4d *vel,
+                        flof_grid4d *phiOrg, flof_grid4d *phiCurr,
+                        flof_grid4d *phiTarget, float threshPhi,
+                        float threshNorm, float postVelBlur,
+                        float resetBndWidth, int maxIter);
+
+/* ---- full-resolution frames (optflow4d.cpp:1445-1595,1622-2105) -------- */
+/* loadPlaceGrid4d with the frame files already decoded into device memory:
+ * `frames_dev` holds (fileIdxEnd - fileIdxStart) frames of sx*sy*sz floats,
+ * frame f at offset f*sx*sy*sz (file index fileIdxStart + f). */
+int flof_load_place_grid4d(const float *frames_dev, int sx, int sy, int sz,
+                           flof_grid4d *phi, const float offset[4],
+                           const float scale[4], int fileIdxStart,
+                           int fileIdxEnd, int debugSkipLoad, float spread,
+                           const float overrideSize[4], float overrideTimeOff,
+                           int overrideGoodRegion, float loadTimeScale,
+                           int rescaleSdfValues, float sdfIsoOff,
+                           float repeatStartFrame);
+/* loadAdvectTimeSlice_OptRun for one deformation volume (useDefoVols off):
+ * the two deformation time slices are read from the device-resident 4D
+ * deformation `defo` instead of a .uni file.  dst is a dx*dy*dz device
+ * buffer; cells outside the bnd=10 / bordSkip band are left unchanged. */
+int flof_advect_time_slice_opt(flof_grid4d *defo, float *dst_dev, int dx,
+                               int dy, int dz, flof_grid4d *phi, float time,
+                               float blendAlpha, float loadTimeScale,
+                               const float defoOffset[4],
+                               const float defoScale[4],
+                               const float defoFactor[4],
+                               const float overrideSize[4],
+                               float overrideTimeOff, int zeroVel,
+                               int bordSkip, float defoAniFac);
+/* Batched form: warps nframes output frames (times time0 + f*dtime) in one
+ * launch; frame f is written at dst_dev + f*dx*dy*dz. */
+int flof_advect_time_slices_opt(flof_grid4d *defo, float *dst_dev, int dx,
+                                int dy, int dz, int nframes, float time0,
+                                float dtime, flof_grid4d *phi,
+                                float blendAlpha, float loadTimeScale,
+                                const float defoOffset[4],
+                                const float defoScale[4],
+                                const float defoFactor[4], int bordSkip,
+                                float defoAniFac);
+
+/* ---- synthetic inputs (no reference counterpart) ----------------------- */
+/* Seeded falling-drop SDF (replaces the FLIP input simulation of
+ * scenes/dataGen2Drop.py): a pool plane at y = 0.2*Y united (min) with
+ * `nspheres` spheres drawn from the splitmix64 stream `seed` -- centre x in
+ * [0.3X,0.7X], start height in [0.6Y,0.8Y], centre z in [0.3Z,0.7Z], radius
+ * rmin*X (or uniform in [rmin,rmax]*X) -- that fall by 0.2*Y*s^2 with
+ * s = t/(T-1); values in cells, clamped to +-clampCells, computed in FP64 and
+ * stored as FP32. */
+int flof_drop_sdf4d(flof_grid4d *phi, uint64_t seed, int nspheres, double rmin,
+                    double rmax, double clampCells);
+/* The same drops as T full-resolution 3D frames at `scale` times the spatial
+ * resolution of an X*Y*Z*T grid, in fine-cell units (clamp clampCells*scale).
+ * Frame t is written at frames_dev + t*(scale^3*X*Y*Z). */
+int flof_drop_frames3d(flof_ctx *ctx, float *frames_dev, int X, int Y, int Z,
+                       int T, int scale, uint64_t seed, int nspheres,
+                       double rmin, double rmax, double clampCells);
+/* sizeof of the ABI structs (0: flof_of_params, 1: flof_solve_info), for
+ * binding checks. */
+size_t flof_abi_struct_size(int which);
+
+/* ---- device memory helpers for 3D frames ------------------------------- */
+int flof_dev_alloc(flof_ctx *ctx, size_t bytes, void **out);
+int flof_dev_free(flof_ctx *ctx, void *ptr);
+int flof_dev_upload(flof_ctx *ctx, void *dst_dev, const void *src,
+                    size_t bytes);
+int flof_dev_download(flof_ctx *ctx, void *dst, const void *src_dev,
+                      size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLOF_H */

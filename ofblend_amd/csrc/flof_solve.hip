@@ -1,0 +1,1225 @@
+// flof_solve.hip -- the 4D optical-flow solve of libflof_hip: matrix-free
+// Jacobi-PCG of the smoothness-regularised system, Gaussian velocity blur,
+// level-set projection correction and the multiscale driver.
+//
+// Reference: source/plugin/optflow4d.cpp -- opticalFlowDim :361-553,
+// GridCGOptflow4d :268-355, gaussianBlurGeneric :127-174, corrVelsOfTempl
+// :613-802, opticalFlowMultiscaleTemplate :936-1173.
+//
+// The system of one level, per interior cell c and component d (unknown
+// U = vel*h, h = 1/X):
+//     (g g^T + (8w + e) I) U(c) - w * sum_{8 nbrs n} U_d(n) = b_d(c)
+// with g = central differences of i1 times X/2 and b = -g (i1 - i0)/dt
+// (+ terms of the incoming flow).  Outer-layer cells are identity rows.  The
+// reference assembles a 4N x 4N matrix (80 B/cell); here the operator is
+// applied from g on the fly with the identical FP32 operation order, so
+// q = A p is bit-identical to the reference's applyMat.
+//
+// PCG per iteration (three kernels, FP64 scalars on the device):
+//   k_cg_apply : q = A p, pq = p.q                      -> alpha = sigma/pq
+//   k_cg_update: x += alpha p, r -= alpha q, max(r), z = r/diag, z.r
+//                                    -> converged? beta = z.r/sigma
+//   k_cg_dir   : p = z + beta p
+// The host enqueues iterations in chunks and reads the device state once per
+// chunk; kernels of iterations past convergence return immediately.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "flof_internal.h"
+
+namespace flof {
+
+struct CgState {
+    double sigma;
+    double alpha;
+    double beta;
+    double acc;      // stopping threshold cgAccuracy * resIni
+    double resIni;   // signed max of the initial residual
+    double resRel;   // (float)(residual / resIni), as ret_residual
+    double r2, b2;   // warm-start residual check
+    int iters;
+    int done;
+    int status;  // 0 converged, 1 not converged, 2 sigma 0/NaN, 3 tiny rhs
+    int maxIter;
+    int warm;    // warm start kept
+    int zero;    // result must be zeroed (failed solve)
+    int pad[2];
+};
+
+struct LevelGeom {
+    Dims4 D;
+    Coord4 C;
+    float offd;   // -wSmooth (times mDx2Inv = 1)
+    float diag;   // 8*wSmooth + wEnergy, formed as the reference does
+    int use_offd; // offd != 0
+    float eps;    // VECTOR_EPSILON of FP32 builds
+};
+
+__device__ __forceinline__ bool interior(const Dims4 &D, int i, int j, int k, int t)
+{
+    return i >= 1 && j >= 1 && k >= 1 && t >= 1 && i < D.X - 1 && j < D.Y - 1 &&
+           k < D.Z - 1 && t < D.T - 1;
+}
+
+// q = A p at cell c (interior).  Order: 8 neighbour terms t-1, z-1, y-1, x-1,
+// x+1, y+1, z+1, t+1, then the 4-wide block row (optflow4d.cpp:211-232).
+__device__ __forceinline__ float4 apply_A(const float4 *__restrict__ p, int64_t c,
+                                          const float4 &pc, const float4 &g,
+                                          const LevelGeom &G)
+{
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (G.use_offd) {
+        const int64_t o[8] = {-G.D.st, -G.D.sz, -(int64_t)G.D.X, -1,
+                              1,       (int64_t)G.D.X, G.D.sz, G.D.st};
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const float4 n = p[c + o[m]];
+            v.x = v.x + G.offd * n.x;
+            v.y = v.y + G.offd * n.y;
+            v.z = v.z + G.offd * n.z;
+            v.w = v.w + G.offd * n.w;
+        }
+    }
+    const float gg[4] = {g.x, g.y, g.z, g.w};
+    const float pp[4] = {pc.x, pc.y, pc.z, pc.w};
+    float out[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            float b = gg[d] * gg[m];
+            if (d == m) b = b + G.diag;
+            out[d] = out[d] + b * pp[m];
+        }
+    }
+    return make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// identity row: 0 + 1*p_d + 0*p_m, exactly as the assembled row evaluates
+__device__ __forceinline__ float4 apply_I(const float4 &pc)
+{
+    const float pp[4] = {pc.x, pc.y, pc.z, pc.w};
+    float out[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        float v = 0.f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) v = v + (d == m ? 1.f : 0.f) * pp[m];
+        out[d] = v;
+    }
+    return make_float4(out[0], out[1], out[2], out[3]);
+}
+
+__device__ __forceinline__ void dd_add4(DD &a, const float4 &u, const float4 &v)
+{
+    dd_add(a, (double)(u.x * v.x));
+    dd_add(a, (double)(u.y * v.y));
+    dd_add(a, (double)(u.z * v.z));
+    dd_add(a, (double)(u.w * v.w));
+}
+
+__device__ __forceinline__ float4 prec_of(const float4 &g, float diag)
+{
+    // precond = 1./blockd[d][d] evaluated in double, stored as float
+    return make_float4((float)(1.0 / (double)(g.x * g.x + diag)),
+                       (float)(1.0 / (double)(g.y * g.y + diag)),
+                       (float)(1.0 / (double)(g.z * g.z + diag)),
+                       (float)(1.0 / (double)(g.w * g.w + diag)));
+}
+
+// --- set-up (optflow4d.cpp:398-493 + the start of solve :284-303) ---------
+__global__ void __launch_bounds__(256)
+k_of_setup(const float *__restrict__ i0, const float *__restrict__ i1,
+           const float4 *__restrict__ vin, float *__restrict__ rhsT,
+           float4 *__restrict__ g, float4 *__restrict__ prec,
+           float4 *__restrict__ x, float4 *__restrict__ r, float4 *__restrict__ p,
+           LevelGeom G, float wSmooth, float wEnergy, float mDt, float mDx,
+           float dxf, DD *part_dd, float *part_max)
+{
+    const Dims4 &D = G.D;
+    const int64_t n = (int64_t)D.st * D.T;
+    DD sig{0., 0.};
+    float mx = -INFINITY;
+    FOR_XCD(c, n)
+    {
+        int i, j, k, t;
+        decompose((uint32_t)c, G.C, i, j, k, t);
+        const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!interior(D, i, j, k, t)) {
+            g[c] = zero4;
+            prec[c] = make_float4(1.f, 1.f, 1.f, 1.f);
+            x[c] = zero4;
+            r[c] = zero4;
+            p[c] = zero4;
+            if (rhsT) rhsT[c] = 0.f;
+            mx = fmaxf(mx, 0.f);
+            continue;
+        }
+        const float tderiv = (i1[c] - i0[c]) / mDt;
+        float4 gc;
+        gc.x = (i1[c + 1] - i1[c - 1]) * dxf;
+        gc.y = (i1[c + D.X] - i1[c - D.X]) * dxf;
+        gc.z = (i1[c + D.sz] - i1[c - D.sz]) * dxf;
+        gc.w = (i1[c + D.st] - i1[c - D.st]) * dxf;
+        float b[4] = {-gc.x * tderiv, -gc.y * tderiv, -gc.z * tderiv, -gc.w * tderiv};
+        if (vin) {
+            const int nbx[8] = {0, 0, 0, -1, 1, 0, 0, 0};
+            const int nby[8] = {0, 0, -1, 0, 0, 1, 0, 0};
+            const int nbz[8] = {0, -1, 0, 0, 0, 0, 1, 0};
+            const int nbt[8] = {-1, 0, 0, 0, 0, 0, 0, 1};
+            const float4 vc = vin[c];
+            const float vcc[4] = {vc.x, vc.y, vc.z, vc.w};
+            for (int m = 0; m < 8; ++m) {
+                const int ti = max(0, min(D.X - 1, i + nbx[m]));
+                const int tj = max(0, min(D.Y - 1, j + nby[m]));
+                const int tk = max(0, min(D.Z - 1, k + nbz[m]));
+                const int tt = max(0, min(D.T - 1, t + nbt[m]));
+                const float4 vn = vin[(int64_t)ti + (int64_t)D.X * tj + D.sz * tk + D.st * tt];
+                const float vnn[4] = {vn.x, vn.y, vn.z, vn.w};
+                for (int d = 0; d < 4; ++d)
+                    b[d] = b[d] - wSmooth * (vcc[d] - vnn[d]) * mDx * 1.f;
+            }
+            for (int d = 0; d < 4; ++d) b[d] = b[d] - wEnergy * vcc[d] * mDx;
+        }
+        const float4 rc = make_float4(b[0], b[1], b[2], b[3]);
+        const float4 pr = prec_of(gc, G.diag);
+        const float4 z = f4mul(rc, pr);
+        g[c] = gc;
+        prec[c] = pr;
+        x[c] = zero4;
+        r[c] = rc;
+        p[c] = z;
+        if (rhsT) rhsT[c] = rc.x;
+        mx = fmaxf(mx, fmaxf(fmaxf(rc.x, rc.y), fmaxf(rc.z, rc.w)));
+        dd_add4(sig, z, rc);
+    }
+    sig = block_dd(sig);
+    mx = block_max(mx);
+    if (threadIdx.x == 0) {
+        part_dd[blockIdx.x] = sig;
+        part_max[blockIdx.x] = mx;
+    }
+}
+
+// reduce partials of nb blocks (one block of 256 threads)
+__device__ __forceinline__ void fin_dd_max(const DD *pd, const float *pm, int nb, DD &s,
+                                           float &m)
+{
+    s = DD{0., 0.};
+    m = -INFINITY;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        dd_merge(s, pd[b]);
+        if (pm) m = fmaxf(m, pm[b]);
+    }
+    s = block_dd(s);
+    m = block_max(m);
+}
+
+__global__ void k_setup_fin(CgState *st, const DD *pd, const float *pm, int nb,
+                            float cgAcc, float eps)
+{
+    DD s;
+    float m;
+    fin_dd_max(pd, pm, nb, s, m);
+    if (threadIdx.x == 0) {
+        const double residual = (double)m;
+        st->iters = 0;
+        st->warm = 0;
+        st->zero = 0;
+        st->resRel = 1e10;
+        st->resIni = residual;
+        if (residual < (double)eps) {
+            st->resRel = 0.;
+            st->status = 3;
+            st->done = 1;
+        } else {
+            st->acc = (double)cgAcc * residual;
+            const double sigma = s.hi + s.lo;
+            st->sigma = sigma;
+            if (sigma == 0. || sigma != sigma) {
+                st->status = 2;
+                st->done = 1;
+            } else {
+                st->status = 1;
+                st->done = 0;
+            }
+        }
+    }
+}
+
+// --- warm start (extension): r_w = b - A x0, ||r_w||^2 and ||b||^2 --------
+__global__ void k_warm_guess(const float4 *__restrict__ warm,
+                             const float4 *__restrict__ vin_acc, float4 *__restrict__ x0s,
+                             Dims4 D, Coord4 C, float mDx)
+{
+    const int64_t n = (int64_t)D.st * D.T;
+    FOR_XCD(c, n)
+    {
+        int i, j, k, t;
+        decompose((uint32_t)c, C, i, j, k, t);
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (interior(D, i, j, k, t)) {
+            float4 w = warm[c];
+            if (vin_acc) w = f4sub(w, vin_acc[c]);
+            v = f4muls(w, mDx);
+        }
+        x0s[c] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_warm_residual(const float4 *__restrict__ x0s, const float4 *__restrict__ g,
+                const float4 *__restrict__ r, float4 *__restrict__ rw, LevelGeom G,
+                DD *part_a, DD *part_b)
+{
+    const Dims4 &D = G.D;
+    const int64_t n = (int64_t)D.st * D.T;
+    DD r2{0., 0.}, b2{0., 0.};
+    FOR_XCD(c, n)
+    {
+        int i, j, k, t;
+        decompose((uint32_t)c, G.C, i, j, k, t);
+        const float4 xc = x0s[c];
+        const float4 ax = interior(D, i, j, k, t) ? apply_A(x0s, c, xc, g[c], G) : apply_I(xc);
+        const float4 b = r[c];
+        const float4 rr = f4sub(b, ax);
+        rw[c] = rr;
+        dd_add(r2, (double)rr.x * rr.x);
+        dd_add(r2, (double)rr.y * rr.y);
+        dd_add(r2, (double)rr.z * rr.z);
+        dd_add(r2, (double)rr.w * rr.w);
+        dd_add(b2, (double)b.x * b.x);
+        dd_add(b2, (double)b.y * b.y);
+        dd_add(b2, (double)b.z * b.z);
+        dd_add(b2, (double)b.w * b.w);
+    }
+    r2 = block_dd(r2);
+    b2 = block_dd(b2);
+    if (threadIdx.x == 0) {
+        part_a[blockIdx.x] = r2;
+        part_b[blockIdx.x] = b2;
+    }
+}
+
+__global__ void k_warm_fin(CgState *st, const DD *pa, const DD *pb, int nb)
+{
+    DD s, u;
+    float m;
+    fin_dd_max(pa, nullptr, nb, s, m);
+    fin_dd_max(pb, nullptr, nb, u, m);
+    if (threadIdx.x == 0) {
+        st->r2 = s.hi + s.lo;
+        st->b2 = u.hi + u.lo;
+        st->warm = (st->done == 0 && st->r2 <= st->b2) ? 1 : 0;
+    }
+}
+
+// adopt the warm start: x = x0s, r = r_w, p = z
+__global__ void __launch_bounds__(256)
+k_warm_adopt(const float4 *__restrict__ x0s, const float4 *__restrict__ rw,
+             const float4 *__restrict__ prec, float4 *__restrict__ x,
+             float4 *__restrict__ r, float4 *__restrict__ p, int64_t n, const CgState *st,
+             DD *part_dd, float *part_max)
+{
+    if (!st->warm) return;
+    DD sig{0., 0.};
+    float mx = -INFINITY;
+    FOR_XCD(c, n)
+    {
+        const float4 rr = rw[c];
+        const float4 z = f4mul(rr, prec[c]);
+        x[c] = x0s[c];
+        r[c] = rr;
+        p[c] = z;
+        mx = fmaxf(mx, fmaxf(fmaxf(rr.x, rr.y), fmaxf(rr.z, rr.w)));
+        dd_add4(sig, z, rr);
+    }
+    sig = block_dd(sig);
+    mx = block_max(mx);
+    if (threadIdx.x == 0) {
+        part_dd[blockIdx.x] = sig;
+        part_max[blockIdx.x] = mx;
+    }
+}
+
+__global__ void k_warm_adopt_fin(CgState *st, const DD *pd, const float *pm, int nb)
+{
+    if (!st->warm) return;
+    DD s;
+    float m;
+    fin_dd_max(pd, pm, nb, s, m);
+    if (threadIdx.x == 0) {
+        const double residual = (double)m;
+        st->resRel = (double)(float)(residual / st->resIni);
+        if (residual <= st->acc) {
+            st->status = 0;
+            st->done = 1;
+        } else {
+            const double sigma = s.hi + s.lo;
+            st->sigma = sigma;
+            if (sigma == 0. || sigma != sigma) {
+                st->status = 2;
+                st->done = 1;
+                st->zero = 1;
+            }
+        }
+    }
+}
+
+// --- PCG iteration kernels -------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_cg_apply(const float4 *__restrict__ p, const float4 *__restrict__ g,
+           float4 *__restrict__ q, LevelGeom G, const CgState *st, DD *part_dd)
+{
+    if (st->done) return;
+    const Dims4 &D = G.D;
+    const int64_t n = (int64_t)D.st * D.T;
+    DD acc{0., 0.};
+    FOR_XCD(c, n)
+    {
+        int i, j, k, t;
+        decompose((uint32_t)c, G.C, i, j, k, t);
+        const float4 pc = p[c];
+        const float4 qc = interior(D, i, j, k, t) ? apply_A(p, c, pc, g[c], G) : apply_I(pc);
+        q[c] = qc;
+        dd_add4(acc, pc, qc);
+    }
+    acc = block_dd(acc);
+    if (threadIdx.x == 0) part_dd[blockIdx.x] = acc;
+}
+
+// alpha = sigma / p.q
+__global__ void k_cg_alpha(CgState *st, const DD *pd, int nb)
+{
+    if (st->done) return;
+    DD s;
+    float m;
+    fin_dd_max(pd, nullptr, nb, s, m);
+    if (threadIdx.x == 0) st->alpha = st->sigma / (s.hi + s.lo);
+}
+
+__global__ void __launch_bounds__(256)
+k_cg_update(float4 *__restrict__ x, float4 *__restrict__ r,
+            const float4 *__restrict__ p, const float4 *__restrict__ q,
+            const float4 *__restrict__ prec, int64_t n, const CgState *st, DD *part_dd,
+            float *part_max)
+{
+    if (st->done) return;
+    const double alpha = st->alpha, nalpha = -alpha;
+    DD sig{0., 0.};
+    float mx = -INFINITY;
+    FOR_XCD(c, n)
+    {
+        const float4 pc = p[c], qc = q[c];
+        float4 xc = x[c], rc = r[c];
+        xc.x = (float)((double)xc.x + alpha * (double)pc.x);
+        xc.y = (float)((double)xc.y + alpha * (double)pc.y);
+        xc.z = (float)((double)xc.z + alpha * (double)pc.z);
+        xc.w = (float)((double)xc.w + alpha * (double)pc.w);
+        rc.x = (float)((double)rc.x + nalpha * (double)qc.x);
+        rc.y = (float)((double)rc.y + nalpha * (double)qc.y);
+        rc.z = (float)((double)rc.z + nalpha * (double)qc.z);
+        rc.w = (float)((double)rc.w + nalpha * (double)qc.w);
+        x[c] = xc;
+        r[c] = rc;
+        const float4 z = f4mul(rc, prec[c]);
+        mx = fmaxf(mx, fmaxf(fmaxf(rc.x, rc.y), fmaxf(rc.z, rc.w)));
+        dd_add4(sig, z, rc);
+    }
+    sig = block_dd(sig);
+    mx = block_max(mx);
+    if (threadIdx.x == 0) {
+        part_dd[blockIdx.x] = sig;
+        part_max[blockIdx.x] = mx;
+    }
+}
+
+// signed-max stopping test, then beta = z.r / sigma (optflow4d.cpp:311-325)
+__global__ void k_cg_beta(CgState *st, const DD *pd, const float *pm, int nb, int it)
+{
+    if (st->done) return;
+    DD s;
+    float m;
+    fin_dd_max(pd, pm, nb, s, m);
+    if (threadIdx.x == 0) {
+        const double residual = (double)m;
+        const double sigmaNew = s.hi + s.lo;
+        st->iters = it + 1;
+        st->resRel = (double)(float)(residual / st->resIni);
+        if (residual <= st->acc) {
+            st->status = 0;
+            st->done = 1;
+        } else if (sigmaNew != sigmaNew || residual != residual) {
+            // a NaN residual ends with a zero result (optflow4d.cpp:509-514)
+            st->status = 2;
+            st->zero = 1;
+            st->done = 1;
+        } else {
+            st->beta = sigmaNew / st->sigma;
+            st->sigma = sigmaNew;
+            if (it + 1 >= st->maxIter) {
+                st->status = 1;
+                st->done = 1;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_cg_dir(float4 *__restrict__ p, const float4 *__restrict__ r,
+         const float4 *__restrict__ prec, int64_t n, const CgState *st)
+{
+    if (st->done) return;
+    const double beta = st->beta;
+    FOR_XCD(c, n)
+    {
+        const float4 z = f4mul(r[c], prec[c]);
+        float4 pc = p[c];
+        pc.x = (float)((double)z.x + beta * (double)pc.x);
+        pc.y = (float)((double)z.y + beta * (double)pc.y);
+        pc.z = (float)((double)z.z + beta * (double)pc.z);
+        pc.w = (float)((double)z.w + beta * (double)pc.w);
+        p[c] = pc;
+    }
+}
+
+// vel_out (=) or (+=) x / mDx; a failed solve contributes zero
+__global__ void k_of_finish(const float4 *__restrict__ x, float4 *__restrict__ out,
+                            int64_t n, float mDx, int accumulate, const CgState *st)
+{
+    const bool zero = st->zero != 0;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!zero) {
+            const float4 xc = x[c];
+            v = make_float4(xc.x / mDx, xc.y / mDx, xc.z / mDx, xc.w / mDx);
+        }
+        out[c] = accumulate ? f4add(out[c], v) : v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gaussian blur (optflow4d.cpp:127-174).  The reference sums a normalised
+// (2s+1)^4 window clipped to the grid; its weights factor per dimension, so
+// one pass is evaluated here as four normalised 1D passes (x, y, z over the
+// whole grid, t writing the interior only).  Rounding differs from the 4D
+// sum by a few ulp.
+template <class T>
+__global__ void k_blur1d(const T *__restrict__ src, T *__restrict__ dst, Dims4 D,
+                         Coord4 C, int axis, int s, const float *__restrict__ w,
+                         int interior_only)
+{
+    const int64_t n = (int64_t)D.st * D.T;
+    const int64_t stride[4] = {1, (int64_t)D.X, D.sz, D.st};
+    const int size[4] = {D.X, D.Y, D.Z, D.T};
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        int ijkt[4];
+        decompose((uint32_t)c, C, ijkt[0], ijkt[1], ijkt[2], ijkt[3]);
+        if (interior_only && !interior(D, ijkt[0], ijkt[1], ijkt[2], ijkt[3])) continue;
+        const int pos = ijkt[axis];
+        T val = VOps<T>::zero();
+        float wsum = 0.f;
+        for (int o = -s; o <= s; ++o) {
+            const int q = pos + o;
+            if (q < 0 || q >= size[axis]) continue;
+            const float wo = w[o + s];
+            wsum = wsum + wo;
+            val = VOps<T>::add(val, VOps<T>::muls(src[c + (int64_t)o * stride[axis]], wo));
+        }
+        dst[c] = VOps<T>::muls(val, 1.f / wsum);
+    }
+}
+
+// one pass: dst interior = G(src); dst boundary untouched
+static void blur_pass(flof_ctx *ctx, const void *src, void *dst, void *t1, void *t2,
+                      const Dims4 &D, bool vec4, int s, const float *wdev)
+{
+    const int64_t n = (int64_t)D.st * D.T;
+    Coord4 C = make_coord4(D);
+    const int nb = grid_blocks(ctx, n);
+    const void *in[4] = {src, t1, t2, t1};
+    void *out[4] = {t1, t2, t1, dst};
+    for (int axis = 0; axis < 4; ++axis) {
+        if (vec4)
+            hipLaunchKernelGGL(k_blur1d<float4>, dim3(nb), dim3(256), 0, ctx->stream,
+                               (const float4 *)in[axis], (float4 *)out[axis], D, C, axis,
+                               s, wdev, axis == 3 ? 1 : 0);
+        else
+            hipLaunchKernelGGL(k_blur1d<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                               (const float *)in[axis], (float *)out[axis], D, C, axis, s,
+                               wdev, axis == 3 ? 1 : 0);
+        check_launch("k_blur1d");
+    }
+}
+
+void gaussian_blur(flof_ctx *ctx, void *a, const Dims4 &D, bool vec4, float sigma,
+                   int iter)
+{
+    int s = int(1. * sigma + 0.5);
+    if (s == 0) s = 1;
+    FLOF_REQUIRE(s <= 64, FLOF_ERR_RANGE, "gaussianBlur: sigma too large");
+    float wh[129];
+    for (int o = -s; o <= s; ++o)
+        wh[o + s] = (float)std::exp(-(double)(float)(o * o) /
+                                    (2. * (double)sigma * (double)sigma));
+    const int64_t n = (int64_t)D.st * D.T;
+    const size_t bytes = (size_t)n * (vec4 ? 16 : 4);
+    Tmp wdev(ctx, 129 * sizeof(float)), b(ctx, bytes), t1(ctx, bytes), t2(ctx, bytes);
+    FLOF_HIP(hipMemcpyAsync(wdev.p, wh, sizeof(wh), hipMemcpyHostToDevice, ctx->stream));
+    launch_fill(ctx, b.p, bytes);
+    for (int it = 0; it < iter; ++it) {
+        // pass 1 into a zeroed temp, pass 2 back into the grid: the grid keeps
+        // its outer layer, the second pass sees a zero outer layer
+        blur_pass(ctx, a, b.p, t1.p, t2.p, D, vec4, s, wdev.as<float>());
+        blur_pass(ctx, b.p, a, t1.p, t2.p, D, vec4, s, wdev.as<float>());
+        if (it + 1 < iter) launch_fill(ctx, b.p, bytes);
+    }
+    // the host weights must outlive the async copy
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// ---------------------------------------------------------------------------
+// level-set projection (optflow4d.cpp:651-734)
+
+__device__ __forceinline__ bool in_bounds_pos(const Dims4 &D, float x, float y, float z,
+                                              float t)
+{
+    const int i = (int)x, j = (int)y, k = (int)z, l = (int)t;
+    return i >= 0 && j >= 0 && k >= 0 && l >= 0 && i < D.X && j < D.Y && k < D.Z &&
+           l < D.T;
+}
+
+__global__ void k_project_cells(float4 *__restrict__ dst, const float4 *__restrict__ vel,
+                                const float *__restrict__ phiOrg,
+                                const float *__restrict__ phiTarget,
+                                float *__restrict__ marker, Dims4 D, Coord4 C,
+                                float threshPhi, int maxIter, float dt)
+{
+    const int64_t n = (int64_t)D.st * D.T;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k, t;
+        decompose((uint32_t)c, C, i, j, k, t);
+        if (i < 2 || j < 2 || k < 2 || t < 2 || i >= D.X - 2 || j >= D.Y - 2 ||
+            k >= D.Z - 2 || t >= D.T - 2)
+            continue;
+        const float4 v = f4muls(vel[c], dt);
+        const float px = ((float)i + 0.5f) - v.x, py = ((float)j + 0.5f) - v.y,
+                    pz = ((float)k + 0.5f) - v.z, pt = ((float)t + 0.5f) - v.w;
+        const float h = 0.5f;
+        float4 nrm;
+        nrm.x = interp4<float>(phiOrg, D, px + h, py, pz, pt) -
+                interp4<float>(phiOrg, D, px - h, py, pz, pt);
+        nrm.y = interp4<float>(phiOrg, D, px, py + h, pz, pt) -
+                interp4<float>(phiOrg, D, px, py - h, pz, pt);
+        nrm.z = interp4<float>(phiOrg, D, px, py, pz + h, pt) -
+                interp4<float>(phiOrg, D, px, py, pz - h, pt);
+        nrm.w = interp4<float>(phiOrg, D, px, py, pz, pt + h) -
+                interp4<float>(phiOrg, D, px, py, pz, pt - h);
+        {
+            const float l = nrm.x * nrm.x + nrm.y * nrm.y + nrm.z * nrm.z + nrm.w * nrm.w;
+            const float e2 = 1e-6f * 1e-6f;
+            if (fabs((double)l - 1.) < (double)e2) {
+            } else if (l > e2) {
+                const float nr = sqrtf(l);
+                nrm = f4muls(nrm, (float)(1. / (double)nr));
+            } else {
+                nrm = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+        // line search along the normal (projectCell, optflow4d.cpp:675-710)
+        float step = 0.25f;
+        if (maxIter < 50) step = 0.5f;
+        int lastDir = 0;
+        const float target = phiTarget[c];
+        float vn = interp4<float>(phiOrg, D, px, py, pz, pt);
+        float d = vn - target;
+        for (int s = 0; s < maxIter; ++s) {
+            const float4 o = f4muls(f4muls(nrm, d), dt);
+            const float qx = px + o.x, qy = py + o.y, qz = pz + o.z, qt = pt + o.w;
+            if (!in_bounds_pos(D, qx, qy, qz, qt)) {
+                s = maxIter;
+                d = 9999.f;
+                continue;
+            }
+            vn = interp4<float>(phiOrg, D, qx, qy, qz, qt);
+            if (vn < target) {
+                if (lastDir < 0) step = (float)(step * 0.5);
+                d += step;
+                lastDir = 1;
+            } else {
+                if (lastDir > 0) step = (float)(step * 0.5);
+                d -= step;
+                lastDir = -1;
+            }
+            if (step < 1e-01) s = maxIter + 1;
+        }
+        if (fabsf(d) > threshPhi) {
+            dst[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
+        }
+        dst[c] = f4muls(nrm, d);
+        marker[c] = 1.f;
+    }
+}
+
+// knCvExpolBlur4d (optflow4d.cpp:613-626): unmarked interior cells take the
+// 3^4 box mean, summed in the reference's loop order
+__global__ void k_fill_box81(const float4 *__restrict__ a, float4 *__restrict__ tmp,
+                             const float *__restrict__ marker, Dims4 D, Coord4 C)
+{
+    const int64_t n = (int64_t)D.st * D.T;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        int i, j, k, t;
+        decompose((uint32_t)c, C, i, j, k, t);
+        if (!interior(D, i, j, k, t)) {
+            tmp[c] = a[c];
+            continue;
+        }
+        if (marker[c] != 0.f) {
+            tmp[c] = a[c];
+            continue;
+        }
+        float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int vt = -1; vt <= 1; ++vt)
+            for (int zk = -1; zk <= 1; ++zk)
+                for (int yj = -1; yj <= 1; ++yj)
+                    for (int xi = -1; xi <= 1; ++xi)
+                        val = f4add(val, a[c + xi + (int64_t)D.X * yj + D.sz * zk + D.st * vt]);
+        const double f = 1. / 81.0;
+        tmp[c] = make_float4((float)(val.x * f), (float)(val.y * f), (float)(val.z * f),
+                             (float)(val.w * f));
+    }
+}
+
+void corr_vels_of(flof_ctx *ctx, float4 *dst, float4 *vel, const float *phiOrg,
+                  const float *phiTarget, const Dims4 &D, float threshPhi,
+                  float postVelBlur, float resetBndWidth, int maxIter, float dt)
+{
+    const int64_t n = (int64_t)D.st * D.T;
+    const float blurThreshold = 0.98f;
+    const int doCvExtrapol = (int)(host_max_norm_vec4(ctx, vel, n) + 4);
+    Tmp marker(ctx, (size_t)n * 4), tmp(ctx, (size_t)n * 16);
+    Coord4 C = make_coord4(D);
+    const int nb = grid_blocks(ctx, n);
+    int singleStep = 0;
+    if (postVelBlur < blurThreshold) singleStep = 1;
+    while ((postVelBlur >= blurThreshold) || singleStep) {
+        launch_fill(ctx, dst, (size_t)n * 16);
+        launch_fill(ctx, marker.p, (size_t)n * 4);
+        hipLaunchKernelGGL(k_project_cells, dim3(nb), dim3(256), 0, ctx->stream, dst, vel,
+                           phiOrg, phiTarget, marker.as<float>(), D, C, threshPhi, maxIter,
+                           dt);
+        check_launch("k_project_cells");
+        for (int it = 0; it < doCvExtrapol; ++it) {
+            hipLaunchKernelGGL(k_fill_box81, dim3(nb), dim3(256), 0, ctx->stream, dst,
+                               tmp.as<float4>(), marker.as<float>(), D, C);
+            check_launch("k_fill_box81");
+            FLOF_HIP(hipMemcpyAsync(dst, tmp.p, (size_t)n * 16, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+        }
+        if (postVelBlur > 1e-6f)
+            gaussian_blur(ctx, dst, D, true, (float)(0.5 * postVelBlur), 1);
+        const int resetBnd = int(resetBndWidth * (float)D.X) + 1;
+        if (resetBndWidth > 0.f)
+            launch_set_bound(ctx, dst, D, true, make_float4(0.f, 0.f, 0.f, 0.f), resetBnd);
+        launch_axpy4(ctx, vel, dst, make_float4(-1.f, -1.f, -1.f, -1.f), n);
+        postVelBlur = (float)(postVelBlur * 0.5);
+        if (singleStep > 0) singleStep--;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one level (opticalFlowDim)
+
+struct SolveOut {
+    int iters = 0, status = 0, warm = 0;
+    double resRel = 0.;
+};
+
+struct LevelWork {
+    Tmp g, prec, x, r, p, q, st, part;
+    int nb = 0;
+    LevelWork(flof_ctx *ctx, int64_t n)
+    {
+        const size_t b = (size_t)n * 16;
+        g = Tmp(ctx, b);
+        prec = Tmp(ctx, b);
+        x = Tmp(ctx, b);
+        r = Tmp(ctx, b);
+        p = Tmp(ctx, b);
+        q = Tmp(ctx, b);
+        nb = grid_blocks_xcd(ctx, n);
+        st = Tmp(ctx, sizeof(CgState));
+        part = Tmp(ctx, (size_t)nb * (2 * sizeof(DD) + sizeof(float)) + 256);
+        FLOF_HIP(hipMemsetAsync(st.p, 0, sizeof(CgState), ctx->stream));
+    }
+    DD *pdd() const { return part.as<DD>(); }
+    DD *pdd2() const { return part.as<DD>() + nb; }
+    float *pmax() const { return (float *)(part.as<DD>() + 2 * nb); }
+};
+
+// Solves one level.  vin: incoming flow for the rhs (may be null = zero).
+// The result x/h is written to (accumulate ? += : =) out.
+SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const float4 *vin,
+                     float *rhsT, const Dims4 &D, float wSmooth, float wEnergy,
+                     float cgAccuracy, float dt, int maxIter, const float4 *warm,
+                     const float4 *warm_base, float4 *out, bool accumulate)
+{
+    FLOF_REQUIRE(D.X >= 1 && D.Y >= 1 && D.Z >= 1 && D.T >= 1, FLOF_ERR_DIM,
+                 "opticalFlow4d: bad size");
+    const int64_t n = (int64_t)D.st * D.T;
+    FLOF_REQUIRE(n < (1ll << 31), FLOF_ERR_DIM, "opticalFlow4d: grid too large");
+    LevelWork W(ctx, n);
+    CgState *st = W.st.as<CgState>();
+    const float mDx = (float)(1. / D.X);
+    const float dxf = (float)(1. / (2. * (double)mDx));
+    LevelGeom G;
+    G.D = D;
+    G.C = make_coord4(D);
+    G.offd = -wSmooth * 1.f;
+    float diag = 0.f;
+    diag += (float)(8) * wSmooth * 1.f;
+    diag += wEnergy;
+    G.diag = diag;
+    G.use_offd = G.offd != 0.f;
+    G.eps = 1e-6f;
+    {
+        int maxIt = maxIter;
+        FLOF_HIP(hipMemcpyAsync(&st->maxIter, &maxIt, sizeof(int), hipMemcpyHostToDevice,
+                                ctx->stream));
+        FLOF_HIP(hipStreamSynchronize(ctx->stream));  // maxIt lives on the stack
+    }
+    // with fewer than 3 cells in a dimension every row is an identity row and
+    // the neighbour offsets are never used
+    const int nb = W.nb;
+    hipLaunchKernelGGL(k_of_setup, dim3(nb), dim3(256), 0, ctx->stream, i0, i1, vin, rhsT,
+                       W.g.as<float4>(), W.prec.as<float4>(), W.x.as<float4>(),
+                       W.r.as<float4>(), W.p.as<float4>(), G, wSmooth, wEnergy, dt, mDx, dxf,
+                       W.pdd(), W.pmax());
+    check_launch("k_of_setup");
+    hipLaunchKernelGGL(k_setup_fin, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
+                       W.pmax(), nb, cgAccuracy, G.eps);
+    check_launch("k_setup_fin");
+    if (warm) {
+        hipLaunchKernelGGL(k_warm_guess, dim3(nb), dim3(256), 0, ctx->stream, warm,
+                           warm_base, W.q.as<float4>(), D, G.C, mDx);
+        check_launch("k_warm_guess");
+        Tmp rw(ctx, (size_t)n * 16);
+        hipLaunchKernelGGL(k_warm_residual, dim3(nb), dim3(256), 0, ctx->stream,
+                           W.q.as<float4>(), W.g.as<float4>(), W.r.as<float4>(),
+                           rw.as<float4>(), G, W.pdd(), W.pdd2());
+        check_launch("k_warm_residual");
+        hipLaunchKernelGGL(k_warm_fin, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
+                           W.pdd2(), nb);
+        check_launch("k_warm_fin");
+        hipLaunchKernelGGL(k_warm_adopt, dim3(nb), dim3(256), 0, ctx->stream,
+                           W.q.as<float4>(), rw.as<float4>(), W.prec.as<float4>(),
+                           W.x.as<float4>(), W.r.as<float4>(), W.p.as<float4>(), n, st,
+                           W.pdd(), W.pmax());
+        check_launch("k_warm_adopt");
+        hipLaunchKernelGGL(k_warm_adopt_fin, dim3(1), dim3(256), 0, ctx->stream, st,
+                           W.pdd(), W.pmax(), nb);
+        check_launch("k_warm_adopt_fin");
+    }
+    CgState hs;
+    int launched = 0, chunk = 4;
+    FLOF_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    while (!hs.done && launched < maxIter) {
+        const int m = std::min(chunk, maxIter - launched);
+        for (int k = 0; k < m; ++k) {
+            const int it = launched + k;
+            {
+                TimedLaunch tl(ctx, FLOF_KT_CG_APPLY);
+                hipLaunchKernelGGL(k_cg_apply, dim3(nb), dim3(256), 0, ctx->stream,
+                                   W.p.as<float4>(), W.g.as<float4>(), W.q.as<float4>(), G,
+                                   st, W.pdd());
+            }
+            hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
+                               nb);
+            {
+                TimedLaunch tl(ctx, FLOF_KT_CG_UPDATE);
+                hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, ctx->stream,
+                                   W.x.as<float4>(), W.r.as<float4>(), W.p.as<float4>(),
+                                   W.q.as<float4>(), W.prec.as<float4>(), n, st, W.pdd(),
+                                   W.pmax());
+            }
+            hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
+                               W.pmax(), nb, it);
+            {
+                TimedLaunch tl(ctx, FLOF_KT_CG_DIR);
+                hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(256), 0, ctx->stream,
+                                   W.p.as<float4>(), W.r.as<float4>(), W.prec.as<float4>(),
+                                   n, st);
+            }
+        }
+        check_launch("k_cg_*");
+        launched += m;
+        chunk = std::min(chunk * 2, 64);
+        FLOF_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost,
+                                ctx->stream));
+        FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    hipLaunchKernelGGL(k_of_finish, dim3(W.nb), dim3(256), 0, ctx->stream, W.x.as<float4>(),
+                       out, n, mDx, accumulate ? 1 : 0, st);
+    check_launch("k_of_finish");
+    SolveOut so;
+    so.iters = hs.iters;
+    so.status = hs.status;
+    so.warm = hs.warm;
+    so.resRel = hs.resRel;
+    return so;
+}
+
+// ---------------------------------------------------------------------------
+// multiscale driver (optflow4d.cpp:936-1173)
+
+struct MsParams {
+    float wSmooth, wEnergy, postVelBlur, cgAccuracy, cfl, resetBndWidth;
+    int multiStep, projSizeThresh, minGridSize, doFinalProject, cgMaxIter;
+};
+
+static void record(flof_solve_info *info, int level, int step, const SolveOut &so)
+{
+    if (!info || level >= FLOF_MAX_LEVELS || step >= FLOF_MAX_STEPS) return;
+    info->iters[level][step] = so.iters;
+    info->residual[level][step] = so.resRel;
+    info->status[level][step] = so.status;
+    info->warm_used[level] = so.warm;
+    if (step + 1 > info->steps[level]) info->steps[level] = step + 1;
+    info->total_iters += so.iters;
+}
+
+// level solve with the post-processing of opticalFlowDim (blur, band reset)
+static SolveOut of_level_post(flof_ctx *ctx, float4 *velCurr, const float *i0w,
+                              const float *i1, float *rhsT, const Dims4 &D, float wSmooth,
+                              float wEnergy, float postVelBlur, float cgAccuracy,
+                              float resetBndWidth, float dt, int maxIter,
+                              const float4 *warm, const float4 *warm_base)
+{
+    SolveOut so = solve_level(ctx, i0w, i1, nullptr, rhsT, D, wSmooth, wEnergy, cgAccuracy,
+                              dt, maxIter, warm, warm_base, velCurr, false);
+    if (postVelBlur > 0.f)
+        gaussian_blur(ctx, velCurr, D, true, (float)(0.5 * postVelBlur), 1);
+    if (resetBndWidth > 0.f) {
+        const int rb = int(resetBndWidth * (float)D.X) + 1;
+        launch_set_bound(ctx, velCurr, D, true, make_float4(0.f, 0.f, 0.f, 0.f), rb - 1);
+    }
+    return so;
+}
+
+static float multiscale(flof_ctx *ctx, float4 *vel, const float *i0, const float *i1,
+                        float *rhsT, const Dims4 &D, const MsParams &P, int level, float dt,
+                        flof_solve_info *info, const float4 *warm)
+{
+    const int64_t n = (int64_t)D.st * D.T;
+    const size_t rb = (size_t)n * 4, vb = (size_t)n * 16;
+    int multiStep = P.multiStep;
+    bool doFinalProject = P.doFinalProject != 0;
+    const int resetBnd = P.resetBndWidth > 0 ? int(P.resetBndWidth * (float)D.X) + 1 : 0;
+    const float projMaxDist = 4.f;
+    const int projMaxIter = 40;
+    const float lsDiffFac = (float)(0.1 / 20.);
+    if (info && level < FLOF_MAX_LEVELS) {
+        info->dims[level][0] = D.X;
+        info->dims[level][1] = D.Y;
+        info->dims[level][2] = D.Z;
+        info->dims[level][3] = D.T;
+        if (level + 1 > info->levels) info->levels = level + 1;
+    }
+    Tmp i0w(ctx, rb);
+    FLOF_HIP(hipMemcpyAsync(i0w.p, i0, rb, hipMemcpyDeviceToDevice, ctx->stream));
+    if (info && level == 0)
+        info->err_pre = host_ls_diff(ctx, i0, i1, nullptr, D, lsDiffFac, resetBnd);
+
+    if (D.X > P.minGridSize) {
+        Dims4 Ds{D.X / 2, D.Y / 2, D.Z / 2, D.T / 2, 0, 0};
+        Ds.sz = (int64_t)Ds.X * Ds.Y;
+        Ds.st = Ds.sz * Ds.Z;
+        FLOF_REQUIRE(Ds.X >= 1 && Ds.Y >= 1 && Ds.Z >= 1 && Ds.T >= 1, FLOF_ERR_DIM,
+                     "opticalFlowMultiscale4d: pyramid level too small");
+        const int64_t ns = (int64_t)Ds.st * Ds.T;
+        Tmp i0s(ctx, (size_t)ns * 4), i1s(ctx, (size_t)ns * 4), vs(ctx, (size_t)ns * 16),
+            dbg(ctx, (size_t)ns * 4), ws;
+        launch_pyramid(ctx, i0s.p, Ds, i0, D, false, 1.f);
+        launch_pyramid(ctx, i1s.p, Ds, i1, D, false, 1.f);
+        launch_pyramid(ctx, vs.p, Ds, vel, D, true, 0.5f);
+        if (warm) {
+            ws = Tmp(ctx, (size_t)ns * 16);
+            launch_pyramid(ctx, ws.p, Ds, warm, D, true, 0.5f);
+        }
+        launch_fill(ctx, dbg.p, (size_t)ns * 4);
+        multiscale(ctx, vs.as<float4>(), i0s.as<float>(), i1s.as<float>(), dbg.as<float>(),
+                   Ds, P, level + 1, dt, info, warm ? ws.as<float4>() : nullptr);
+        launch_pyramid(ctx, vel, D, vs.p, Ds, true, 2.f);
+        if (rhsT) FLOF_HIP(hipMemcpyAsync(rhsT, i0w.p, rb, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+
+    advect_cfl(ctx, vel, i0w.p, D, false, P.cfl, 1.f, dt);
+    launch_neumann(ctx, i0w.p, D, false, 0);
+    double errCurr = 0.;
+    if (info || multiStep > 1)
+        errCurr = host_ls_diff(ctx, i0w.as<float>(), i1, nullptr, D, lsDiffFac, resetBnd);
+    if (info && level < FLOF_MAX_LEVELS) info->err_level[level] = errCurr;
+
+    bool doProject = false;
+    if (D.X > P.projSizeThresh) {
+        doProject = true;
+        multiStep = 1;
+        doFinalProject = false;
+    }
+
+    if (multiStep > 1) {
+        FLOF_REQUIRE(multiStep < FLOF_MAX_STEPS, FLOF_ERR_RANGE, "Too many of substeps!");
+        std::vector<Tmp> vsv, vs2;
+        for (int s = 0; s < multiStep; ++s) {
+            vs2.emplace_back(ctx, vb);
+            launch_fill(ctx, vs2.back().p, vb);
+        }
+        for (int s = 0; s < multiStep; ++s) vsv.emplace_back();
+        Tmp tmpVel(ctx, vb);
+        float velBlur = P.postVelBlur;
+        double errLast = errCurr;
+        int steps = multiStep;
+        for (int of = 0; of < steps; ++of) {
+            vsv[of] = Tmp(ctx, vb);
+            SolveOut so = of_level_post(ctx, vsv[of].as<float4>(), i0w.as<float>(), i1, rhsT,
+                                        D, P.wSmooth, P.wEnergy, velBlur, P.cgAccuracy,
+                                        P.resetBndWidth, dt, P.cgMaxIter, nullptr, nullptr);
+            record(info, level, of, so);
+            velBlur = (float)(velBlur * (3. / 4.));
+            if (velBlur < 2.f) velBlur = 2.f;
+            for (int k = of; k >= 0; --k)
+                FLOF_HIP(hipMemcpyAsync(vs2[k].p, vsv[k].p, vb, hipMemcpyDeviceToDevice,
+                                        ctx->stream));
+            for (int k = of - 1; k >= 0; --k)
+                for (int l = 0; l < k; ++l)
+                    advect_cfl(ctx, vs2[k].as<float4>(), vs2[l].p, D, true, P.cfl, 1.f, dt);
+            FLOF_HIP(hipMemcpyAsync(tmpVel.p, vel, vb, hipMemcpyDeviceToDevice, ctx->stream));
+            for (int k = of; k >= 0; --k)
+                launch_axpy4(ctx, tmpVel.as<float4>(), vs2[k].as<float4>(),
+                             make_float4(1.f, 1.f, 1.f, 1.f), n);
+            Tmp w2(ctx, rb);
+            FLOF_HIP(hipMemcpyAsync(w2.p, i0, rb, hipMemcpyDeviceToDevice, ctx->stream));
+            advect_cfl(ctx, tmpVel.as<float4>(), w2.p, D, false, P.cfl, 1.f, dt);
+            launch_neumann(ctx, w2.p, D, false, 0);
+            const double errC =
+                host_ls_diff(ctx, w2.as<float>(), i1, nullptr, D, lsDiffFac, resetBnd);
+            FLOF_HIP(hipMemcpyAsync(i0w.p, w2.p, rb, hipMemcpyDeviceToDevice, ctx->stream));
+            if (of > 0 && ((float)errC / (float)errLast) > 0.95) {
+                launch_fill(ctx, vsv[of].p, vb);
+                steps = of + 1;
+            }
+            errLast = errC;
+        }
+        for (int of = steps - 1; of >= 0; --of)
+            for (int l = 0; l < of; ++l)
+                advect_cfl(ctx, vsv[of].as<float4>(), vsv[l].p, D, true, P.cfl, 1.f, dt);
+        for (int of = 0; of < steps; ++of)
+            launch_axpy4(ctx, vel, vsv[of].as<float4>(), make_float4(1.f, 1.f, 1.f, 1.f), n);
+    } else if (!doProject) {
+        const bool post = P.postVelBlur > 0.f || P.resetBndWidth > 0.f;
+        if (!post) {
+            // vel += x/h fused into the finish kernel
+            SolveOut so = solve_level(ctx, i0w.as<float>(), i1, nullptr, rhsT, D, P.wSmooth,
+                                      P.wEnergy, P.cgAccuracy, dt, P.cgMaxIter, warm, vel,
+                                      vel, true);
+            record(info, level, 0, so);
+        } else {
+            Tmp velCurr(ctx, vb);
+            SolveOut so = of_level_post(ctx, velCurr.as<float4>(), i0w.as<float>(), i1, rhsT,
+                                        D, P.wSmooth, P.wEnergy, P.postVelBlur, P.cgAccuracy,
+                                        P.resetBndWidth, dt, P.cgMaxIter, warm, vel);
+            record(info, level, 0, so);
+            launch_axpy4(ctx, vel, velCurr.as<float4>(), make_float4(1.f, 1.f, 1.f, 1.f), n);
+        }
+    } else {
+        Tmp velCurr(ctx, vb), velTmp2(ctx, vb);
+        launch_fill(ctx, velTmp2.p, vb);
+        corr_vels_of(ctx, velCurr.as<float4>(), velTmp2.as<float4>(), i0w.as<float>(), i1, D,
+                     projMaxDist, P.postVelBlur, P.resetBndWidth, projMaxIter, dt);
+        launch_axpy4(ctx, vel, velTmp2.as<float4>(), make_float4(1.f, 1.f, 1.f, 1.f), n);
+        launch_scale4(ctx, velCurr.as<float4>(), make_float4(-1.f, -1.f, -1.f, -1.f), n);
+        launch_axpy4(ctx, vel, velCurr.as<float4>(), make_float4(1.f, 1.f, 1.f, 1.f), n);
+    }
+
+    if (level == 0 && doFinalProject) {
+        Tmp velCurr(ctx, vb);
+        corr_vels_of(ctx, velCurr.as<float4>(), vel, i0, i1, D, projMaxDist, 4.f,
+                     P.resetBndWidth, projMaxIter, dt);
+    }
+    if (level == 0) {
+        FLOF_HIP(hipMemcpyAsync(i0w.p, i0, rb, hipMemcpyDeviceToDevice, ctx->stream));
+        advect_cfl(ctx, vel, i0w.p, D, false, P.cfl, 1.f, dt);
+        launch_neumann(ctx, i0w.p, D, false, 0);
+        errCurr = host_ls_diff(ctx, i0w.as<float>(), i1, nullptr, D, lsDiffFac, resetBnd);
+        if (info) info->err_final = errCurr;
+    }
+    return (float)errCurr;
+}
+
+}  // namespace flof
+
+using namespace flof;
+
+#define API_TRY try {
+#define API_CATCH(ctx)                                                        \
+    }                                                                         \
+    catch (const flof::Error &e)                                              \
+    {                                                                         \
+        flof::set_error(ctx, e);                                              \
+        return e.code;                                                        \
+    }                                                                         \
+    catch (const std::exception &e)                                           \
+    {                                                                         \
+        flof::set_error(ctx, flof::Error{FLOF_ERR_ARG, e.what()});            \
+        return FLOF_ERR_ARG;                                                  \
+    }                                                                         \
+    return FLOF_OK;
+
+static void need4(const flof_grid4d *g, int elem, const char *what)
+{
+    FLOF_REQUIRE(g, FLOF_ERR_ARG, std::string(what) + ": null grid");
+    FLOF_REQUIRE(g->dtype == FLOF_F32, FLOF_ERR_TYPE, std::string(what) + ": FP32 storage");
+    FLOF_REQUIRE(g->elem == elem, FLOF_ERR_TYPE, std::string(what) + ": wrong element type");
+}
+static void need_dims(const flof_grid4d *a, const flof_grid4d *b, const char *what)
+{
+    FLOF_REQUIRE(a->X == b->X && a->Y == b->Y && a->Z == b->Z && a->T == b->T,
+                 FLOF_ERR_DIM, std::string(what) + ": grid sizes differ");
+}
+
+extern "C" {
+
+int flof_gaussian_blur4d(flof_grid4d *grid, float sigma, int iter)
+{
+    flof_ctx *ctx = grid ? grid->ctx : nullptr;
+    API_TRY
+    FLOF_REQUIRE(grid && grid->dtype == FLOF_F32 && grid->elem != FLOF_INT, FLOF_ERR_TYPE,
+                 "gaussianBlur: REAL or VEC4 FP32 grid");
+    gaussian_blur(ctx, grid->data, dims_of(grid), grid->elem == FLOF_VEC4, sigma, iter);
+    API_CATCH(ctx)
+}
+
+int flof_optical_flow4d(flof_grid4d *vel, flof_grid4d *i0, flof_grid4d *i1,
+                        flof_grid4d *rhsT, const flof_of_params *p, flof_solve_info *info)
+{
+    flof_ctx *ctx = vel ? vel->ctx : nullptr;
+    API_TRY
+    need4(vel, FLOF_VEC4, "opticalFlow4d(vel)");
+    need4(i0, FLOF_REAL, "opticalFlow4d(i0)");
+    need4(i1, FLOF_REAL, "opticalFlow4d(i1)");
+    need_dims(vel, i0, "opticalFlow4d");
+    need_dims(vel, i1, "opticalFlow4d");
+    if (rhsT) {
+        need4(rhsT, FLOF_REAL, "opticalFlow4d(rhsT)");
+        need_dims(vel, rhsT, "opticalFlow4d");
+    }
+    FLOF_REQUIRE(p, FLOF_ERR_ARG, "opticalFlow4d: null params");
+    FLOF_REQUIRE(p->blurType == 1 || p->postVelBlur <= 0., FLOF_ERR_ARG, "NYI");
+    Dims4 D = dims_of(vel);
+    const int64_t n = vel->n();
+    // the incoming flow feeds the rhs; the solution replaces it
+    Tmp vin(ctx, (size_t)n * 16);
+    FLOF_HIP(hipMemcpyAsync(vin.p, vel->data, (size_t)n * 16, hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    SolveOut so = solve_level(ctx, (const float *)i0->data, (const float *)i1->data,
+                              vin.as<float4>(), rhsT ? (float *)rhsT->data : nullptr, D,
+                              (float)p->wSmooth, (float)p->wEnergy, (float)p->cgAccuracy, 1.f,
+                              p->cgMaxIter, nullptr, nullptr, (float4 *)vel->data, false);
+    if (p->postVelBlur > 0.)
+        gaussian_blur(ctx, vel->data, D, true, (float)(0.5 * (float)p->postVelBlur), 1);
+    if (p->resetBndWidth > 0.) {
+        const int rb = int((float)p->resetBndWidth * (float)D.X) + 1;
+        launch_set_bound(ctx, vel->data, D, true, make_float4(0.f, 0.f, 0.f, 0.f), rb - 1);
+    }
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->levels = 1;
+        info->dims[0][0] = D.X;
+        info->dims[0][1] = D.Y;
+        info->dims[0][2] = D.Z;
+        info->dims[0][3] = D.T;
+        record(info, 0, 0, so);
+    }
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    API_CATCH(ctx)
+}
+
+int flof_optical_flow_multiscale4d(flof_grid4d *vel, flof_grid4d *i0, flof_grid4d *i1,
+                                   flof_grid4d *rhsT, const flof_of_params *p,
+                                   flof_grid4d *warm, flof_solve_info *info, float *err)
+{
+    flof_ctx *ctx = vel ? vel->ctx : nullptr;
+    API_TRY
+    need4(vel, FLOF_VEC4, "opticalFlowMultiscale4d(vel)");
+    need4(i0, FLOF_REAL, "opticalFlowMultiscale4d(i0)");
+    need4(i1, FLOF_REAL, "opticalFlowMultiscale4d(i1)");
+    need_dims(vel, i0, "opticalFlowMultiscale4d");
+    need_dims(vel, i1, "opticalFlowMultiscale4d");
+    if (rhsT) {
+        need4(rhsT, FLOF_REAL, "opticalFlowMultiscale4d(rhsT)");
+        need_dims(vel, rhsT, "opticalFlowMultiscale4d");
+    }
+    FLOF_REQUIRE(p, FLOF_ERR_ARG, "opticalFlowMultiscale4d: null params");
+    FLOF_REQUIRE(p->blurType == 1 || p->postVelBlur <= 0., FLOF_ERR_ARG, "NYI");
+    if (p->warmStart) {
+        FLOF_REQUIRE(warm, FLOF_ERR_ARG, "opticalFlowMultiscale4d: warmStart without warm grid");
+        need4(warm, FLOF_VEC4, "opticalFlowMultiscale4d(warm)");
+        need_dims(vel, warm, "opticalFlowMultiscale4d");
+    }
+    MsParams P;
+    P.wSmooth = (float)p->wSmooth;
+    P.wEnergy = (float)p->wEnergy;
+    P.postVelBlur = (float)p->postVelBlur;
+    P.cgAccuracy = (float)p->cgAccuracy;
+    P.cfl = (float)p->cfl;
+    P.resetBndWidth = (float)p->resetBndWidth;
+    P.multiStep = p->multiStep;
+    P.projSizeThresh = p->projSizeThresh;
+    P.minGridSize = p->minGridSize;
+    P.doFinalProject = p->doFinalProject;
+    P.cgMaxIter = p->cgMaxIter;
+    FLOF_REQUIRE(P.multiStep < FLOF_MAX_STEPS, FLOF_ERR_RANGE, "Too many of substeps!");
+    flof_solve_info local;
+    flof_solve_info *inf = info ? info : &local;
+    std::memset(inf, 0, sizeof(*inf));
+    const float e = multiscale(ctx, (float4 *)vel->data, (const float *)i0->data,
+                               (const float *)i1->data, rhsT ? (float *)rhsT->data : nullptr,
+                               dims_of(vel), P, p->level, 1.f, inf,
+                               p->warmStart ? (const float4 *)warm->data : nullptr);
+    if (err) *err = e;
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    API_CATCH(ctx)
+}
+
+int flof_corr_vels_of4d(flof_grid4d *dst, flof_grid4d *vel, flof_grid4d *phiOrg,
+                        flof_grid4d *phiCurr, flof_grid4d *phiTarget, float threshPhi,
+                        float threshNorm, float postVelBlur, float resetBndWidth, int maxIter)
+{
+    flof_ctx *ctx = vel ? vel->ctx : nullptr;
+    (void)phiCurr;
+    (void)threshNorm;  // unused by the reference as well (optflow4d.cpp:736,744)
+    API_TRY
+    need4(dst, FLOF_VEC4, "corrVelsOf4d(dst)");
+    need4(vel, FLOF_VEC4, "corrVelsOf4d(vel)");
+    need4(phiOrg, FLOF_REAL, "corrVelsOf4d(phiOrg)");
+    need4(phiTarget, FLOF_REAL, "corrVelsOf4d(phiTarget)");
+    need_dims(vel, dst, "corrVelsOf4d");
+    need_dims(vel, phiOrg, "corrVelsOf4d");
+    need_dims(vel, phiTarget, "corrVelsOf4d");
+    corr_vels_of(ctx, (float4 *)dst->data, (float4 *)vel->data, (const float *)phiOrg->data,
+                 (const float *)phiTarget->data, dims_of(vel), threshPhi, postVelBlur,
+                 resetBndWidth, maxIter, 1.f);
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    API_CATCH(ctx)
+}
+
+}  // extern "C"

@@ -1,0 +1,454 @@
+"""Python mirror of the mantaflow plugin surface that the FlOF path uses
+(scenes/flof.py, tools/tests/test_0032_grid4dop.py, test_0042_interpol4d.py),
+backed by libflof_hip through its C-ABI.
+
+Names, keyword arguments and defaults follow the reference's PYTHON()
+functions in source/plugin/optflow4d.cpp and source/grid4d.{h,cpp}; failures
+raise RuntimeError (FlofError), as the reference's generated wrappers do
+(source/pwrapper/pclass.cpp:50-54).  Grids live in GPU memory; to_numpy() /
+from_numpy() copy them to and from host arrays of shape (T, Z, Y, X[, 4]).
+
+    s  = Solver(gridSize=vec3(64, 64, 64), dim=3, fourthDim=64)
+    i0 = s.create(Grid4Real); i1 = s.create(Grid4Real)
+    vel = s.create(Grid4Vec4)
+    opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, wSmooth=0.1, wEnergy=1e-4,
+                            cgAccuracy=1e-5, minGridSize=16)
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .abi import (FLOF_F32, FLOF_INT, FLOF_REAL, FLOF_VEC4, OfParams,
+                  SolveInfo, default_params)
+from ._lib import FlofError, check, floats
+
+
+class vec3(tuple):
+    def __new__(cls, x=0, y=0, z=0):
+        return tuple.__new__(cls, (x, y, z))
+
+    x = property(lambda s: s[0])
+    y = property(lambda s: s[1])
+    z = property(lambda s: s[2])
+
+
+class Vec4(tuple):
+    def __new__(cls, x=0, y=None, z=None, t=None):
+        if y is None:
+            y = z = t = x
+        return tuple.__new__(cls, (x, y, z, t))
+
+    x = property(lambda s: s[0])
+    y = property(lambda s: s[1])
+    z = property(lambda s: s[2])
+    t = property(lambda s: s[3])
+
+
+vec4 = Vec4
+
+_CTX = None
+
+
+def context(device=0):
+    """The process-wide device context (one per process, as one solver
+    process of the reference)."""
+    global _CTX
+    if _CTX is None:
+        p = C.c_void_p()
+        rc = _lib.lib().flof_ctx_create(C.byref(p), device)
+        if rc != 0:
+            raise FlofError(rc, "could not create a device context (is a GPU "
+                                "visible?)")
+        _CTX = p
+    return _CTX
+
+
+class Solver:
+    """FluidSolver (source/fluidsolver.h:31): owns the 3D grid size, the 4th
+    dimension and the timestep (dt = 1 by default, fluidsolver.cpp:130)."""
+
+    def __init__(self, name="solver", gridSize=vec3(32, 32, 32), dim=3,
+                 fourthDim=-1, device=0):
+        if dim != 3:
+            raise FlofError(-2, "4D grids need a three-dimensional solver")
+        if fourthDim is None or fourthDim < 1:
+            raise FlofError(-2, "Solver does not support 4d grids")
+        self.name = name
+        self.gridSize = tuple(int(v) for v in gridSize)
+        self.fourthDim = int(fourthDim)
+        self.timestep = 1.0
+        self.ctx = context(device)
+
+    def create(self, cls, name=""):
+        return cls(self)
+
+    def getDt(self):
+        return self.timestep
+
+    def step(self):
+        pass
+
+
+class Grid4dBase:
+    ELEM = FLOF_REAL
+    NP_SHAPE_TAIL = ()
+    NP_DTYPE = np.float32
+
+    def __init__(self, parent, size=None):
+        self.parent = parent
+        self.ctx = parent.ctx
+        X, Y, Z = parent.gridSize if size is None else size[:3]
+        T = parent.fourthDim if size is None else size[3]
+        self.size = (int(X), int(Y), int(Z), int(T))
+        h = C.c_void_p()
+        check(_lib.lib().flof_grid4d_create(self.ctx, X, Y, Z, T, self.ELEM,
+                                            FLOF_F32, C.byref(h)), self.ctx)
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().flof_grid4d_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    # sizes (grid4d.h:38-56)
+    def getSizeX(self):
+        return self.size[0]
+
+    def getSizeY(self):
+        return self.size[1]
+
+    def getSizeZ(self):
+        return self.size[2]
+
+    def getSizeT(self):
+        return self.size[3]
+
+    def getSize(self):
+        return Vec4(*self.size)
+
+    def np_shape(self):
+        X, Y, Z, T = self.size
+        return (T, Z, Y, X) + self.NP_SHAPE_TAIL
+
+    def nbytes(self):
+        return int(np.prod(self.np_shape())) * 4
+
+    # host transfer
+    def from_numpy(self, a):
+        a = np.ascontiguousarray(a, dtype=self.NP_DTYPE)
+        if a.shape != self.np_shape():
+            raise FlofError(-2, "shape %s != %s" % (a.shape, self.np_shape()))
+        check(_lib.lib().flof_grid4d_upload(self.h, a.ctypes.data_as(C.c_void_p),
+                                            a.nbytes), self.ctx)
+        return self
+
+    def to_numpy(self):
+        a = np.empty(self.np_shape(), dtype=self.NP_DTYPE)
+        check(_lib.lib().flof_grid4d_download(self.h, a.ctypes.data_as(C.c_void_p),
+                                              a.nbytes), self.ctx)
+        return a
+
+    def _val(self, v):
+        return floats(v)
+
+    # Grid4d methods (grid4d.h:130-247)
+    def clear(self):
+        check(_lib.lib().flof_grid4d_clear(self.h), self.ctx)
+
+    def copyFrom(self, a, copyType=True):
+        check(_lib.lib().flof_grid4d_copy_from(self.h, a.h), self.ctx)
+        return self
+
+    def swap(self, other):
+        check(_lib.lib().flof_grid4d_swap(self.h, other.h), self.ctx)
+
+    def add(self, a):
+        check(_lib.lib().flof_grid4d_add(self.h, a.h), self.ctx)
+
+    def sub(self, a):
+        check(_lib.lib().flof_grid4d_sub(self.h, a.h), self.ctx)
+
+    def mult(self, a):
+        check(_lib.lib().flof_grid4d_mult(self.h, a.h), self.ctx)
+
+    def setConst(self, s):
+        check(_lib.lib().flof_grid4d_set_const(self.h, self._val(s)), self.ctx)
+
+    def addConst(self, s):
+        check(_lib.lib().flof_grid4d_add_const(self.h, self._val(s)), self.ctx)
+
+    def multConst(self, s):
+        check(_lib.lib().flof_grid4d_mult_const(self.h, self._val(s)), self.ctx)
+
+    def addScaled(self, a, factor):
+        check(_lib.lib().flof_grid4d_add_scaled(self.h, a.h, self._val(factor)),
+              self.ctx)
+
+    def clamp(self, min, max):
+        check(_lib.lib().flof_grid4d_clamp(self.h, min, max), self.ctx)
+
+    def _red(self, fn):
+        out = C.c_float()
+        check(fn(self.h, C.byref(out)), self.ctx)
+        return out.value
+
+    def getMax(self):
+        return self._red(_lib.lib().flof_grid4d_get_max)
+
+    def getMin(self):
+        return self._red(_lib.lib().flof_grid4d_get_min)
+
+    def getMaxAbs(self):
+        return self._red(_lib.lib().flof_grid4d_get_max_abs)
+
+    getMaxValue = getMax
+    getMinValue = getMin
+    getMaxAbsValue = getMaxAbs
+
+    def setBound(self, value, boundaryWidth=1):
+        check(_lib.lib().flof_grid4d_set_bound(self.h, self._val(value),
+                                               boundaryWidth), self.ctx)
+
+    def setBoundNeumann(self, boundaryWidth=1):
+        check(_lib.lib().flof_grid4d_set_bound_neumann(self.h, boundaryWidth),
+              self.ctx)
+
+
+class Grid4Real(Grid4dBase):
+    ELEM = FLOF_REAL
+
+
+class Grid4Int(Grid4dBase):
+    ELEM = FLOF_INT
+    NP_DTYPE = np.int32
+
+
+class Grid4Vec4(Grid4dBase):
+    ELEM = FLOF_VEC4
+    NP_SHAPE_TAIL = (4,)
+
+    def _val(self, v):
+        if np.isscalar(v):
+            v = (v, v, v, v)
+        return floats(*v)
+
+
+_LAST_INFO = SolveInfo()
+
+
+def getLastSolveInfo():
+    """Convergence record of the last opticalFlow[Multiscale]4d call."""
+    return _LAST_INFO
+
+
+def _params(**kw):
+    return default_params(**kw)
+
+
+# ---------------------------------------------------------------------------
+# plugin functions (source/plugin/optflow4d.cpp, source/grid4d.cpp)
+
+def opticalFlowMultiscale4d(vel, i0, i1, rhsT=None, wSmooth=0., wEnergy=0.,
+                            level=0, postVelBlur=0., cgAccuracy=1e-04,
+                            blurType=1, cfl=999, orderTime=1, orderSpace=1,
+                            resetBndWidth=-1., multiStep=1, projSizeThresh=9999,
+                            minGridSize=10, doFinalProject=False, warm=None,
+                            cgMaxIter=1000):
+    """optflow4d.cpp:2182-2195.  `warm` (extension) is an initial guess of
+    the total flow for the CG of every level."""
+    if i0.parent.getDt() != 1.0:
+        raise FlofError(-6, "Invalid, only dt 1 for now!")
+    p = _params(wSmooth=wSmooth, wEnergy=wEnergy, level=level,
+                postVelBlur=postVelBlur, cgAccuracy=cgAccuracy,
+                blurType=blurType, cfl=cfl, orderTime=orderTime,
+                orderSpace=orderSpace, resetBndWidth=resetBndWidth,
+                multiStep=multiStep, projSizeThresh=projSizeThresh,
+                minGridSize=minGridSize, doFinalProject=int(doFinalProject),
+                cgMaxIter=cgMaxIter, warmStart=int(warm is not None))
+    err = C.c_float()
+    check(_lib.lib().flof_optical_flow_multiscale4d(
+        vel.h, i0.h, i1.h, rhsT.h if rhsT is not None else None, C.byref(p),
+        warm.h if warm is not None else None, C.byref(_LAST_INFO),
+        C.byref(err)), vel.ctx)
+    return err.value
+
+
+def opticalFlow4d(vel, i0, i1, rhsT=None, wSmooth=0., wEnergy=0.,
+                  postVelBlur=0., cgAccuracy=1e-04, blurType=1,
+                  resetBndWidth=-1.):
+    """optflow4d.cpp:2110-2119."""
+    p = _params(wSmooth=wSmooth, wEnergy=wEnergy, postVelBlur=postVelBlur,
+                cgAccuracy=cgAccuracy, blurType=blurType,
+                resetBndWidth=resetBndWidth)
+    check(_lib.lib().flof_optical_flow4d(
+        vel.h, i0.h, i1.h, rhsT.h if rhsT is not None else None, C.byref(p),
+        C.byref(_LAST_INFO)), vel.ctx)
+
+
+def corrVelsOf4d(dst, vel, phiOrg, phiCurr, phiTarget, threshPhi=1e10,
+                 threshNorm=1e10, postVelBlur=0., resetBndWidth=-1.,
+                 maxIter=100):
+    """optflow4d.cpp:2121-2130."""
+    check(_lib.lib().flof_corr_vels_of4d(dst.h, vel.h, phiOrg.h, phiCurr.h,
+                                         phiTarget.h, threshPhi, threshNorm,
+                                         postVelBlur, resetBndWidth, maxIter),
+          vel.ctx)
+
+
+def advect4d(vel, grid, dtFac=1.):
+    """optflow4d.cpp:1292-1306 (the grid's solver timestep times dtFac)."""
+    check(_lib.lib().flof_advect4d(vel.h, grid.h, grid.parent.getDt(), dtFac),
+          vel.ctx)
+
+
+def calcLsDiff4d(i0, i1, out=None, correction=1., bnd=0):
+    """optflow4d.cpp:2132-2137."""
+    r = C.c_float()
+    check(_lib.lib().flof_calc_ls_diff4d(i0.h, i1.h,
+                                         out.h if out is not None else None,
+                                         correction, bnd, C.byref(r)), i0.ctx)
+    return r.value
+
+
+def interpolateGrid4d(target, source, offset=Vec4(0.), scale=Vec4(1.),
+                      size=Vec4(-1.)):
+    """grid4d.cpp:539-547."""
+    check(_lib.lib().flof_interpolate_grid4d(target.h, source.h,
+                                             floats(*offset), floats(*scale),
+                                             floats(*size)), target.ctx)
+
+
+interpolateGrid4dVec = interpolateGrid4d
+
+
+def gaussianBlur4d(grid, sigma=1., iter=1):
+    """gaussianBlurGeneric (optflow4d.cpp:160-174) on a 4D grid."""
+    check(_lib.lib().flof_gaussian_blur4d(grid.h, sigma, iter), grid.ctx)
+
+
+def setRegion4d(dst, start, end, value):
+    """grid4d.cpp:476-483."""
+    check(_lib.lib().flof_set_region4d(dst.h, floats(*start), floats(*end),
+                                       floats(value)), dst.ctx)
+
+
+def setRegion4dVec4(dst, start, end, value):
+    check(_lib.lib().flof_set_region4d(dst.h, floats(*start), floats(*end),
+                                       floats(*value)), dst.ctx)
+
+
+def grid4dMaxDiff(g1, g2):
+    """grid4d.cpp:419-428 (Vec4 grids: grid4dMaxDiffVec4, :451-464)."""
+    r = C.c_double()
+    check(_lib.lib().flof_grid4d_max_diff(g1.h, g2.h, C.byref(r)), g1.ctx)
+    return r.value
+
+
+grid4dMaxDiffVec4 = grid4dMaxDiff
+
+
+class DeviceArray:
+    """A float32 device buffer for 3D grids / frame stacks."""
+
+    def __init__(self, shape, ctx=None):
+        self.ctx = ctx if ctx is not None else context()
+        self.shape = tuple(int(s) for s in shape)
+        self.nbytes = int(np.prod(self.shape)) * 4
+        p = C.c_void_p()
+        check(_lib.lib().flof_dev_alloc(self.ctx, self.nbytes, C.byref(p)),
+              self.ctx)
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a, ctx=None):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        d = cls(a.shape, ctx)
+        check(_lib.lib().flof_dev_upload(d.ctx, d.ptr,
+                                         a.ctypes.data_as(C.c_void_p), a.nbytes),
+              d.ctx)
+        return d
+
+    def to_numpy(self):
+        a = np.empty(self.shape, dtype=np.float32)
+        check(_lib.lib().flof_dev_download(self.ctx, a.ctypes.data_as(C.c_void_p),
+                                           self.ptr, self.nbytes), self.ctx)
+        return a
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            try:
+                _lib.lib().flof_dev_free(self.ctx, self.ptr)
+            except Exception:
+                pass
+            self.ptr = None
+
+
+def getSliceFrom4d(src, srct, dst):
+    """grid4d.cpp:486-499; dst is a DeviceArray of shape (Z, Y, X)."""
+    dz, dy, dx = dst.shape
+    check(_lib.lib().flof_get_slice_from4d(src.h, srct, dst.ptr, dx, dy, dz),
+          src.ctx)
+
+
+def placeGrid3d(src, dst, dstt):
+    """grid4d.cpp:516-524; src is a DeviceArray of shape (Z, Y, X)."""
+    sz, sy, sx = src.shape
+    check(_lib.lib().flof_place_grid3d(src.ptr, sx, sy, sz, dst.h, dstt),
+          dst.ctx)
+
+
+def loadPlaceGrid4dFrames(frames, phi, offset, scale, fileIdxStart=-1,
+                          fileIdxEnd=-1, debugSkipLoad=999999, spread=1.,
+                          overrideSize=Vec4(-1.), overrideTimeOff=0.,
+                          overrideGoodRegion=0, loadTimeScale=1.,
+                          rescaleSdfValues=False, sdfIsoOff=0.,
+                          repeatStartFrame=0.):
+    """loadPlaceGrid4d (optflow4d.cpp:1464-1595) with the frame files
+    already decoded: `frames` is a DeviceArray (F, sz, sy, sx), frame f being
+    file index max(fileIdxStart, 0) + f."""
+    F, sz, sy, sx = frames.shape
+    check(_lib.lib().flof_load_place_grid4d(
+        frames.ptr, sx, sy, sz, phi.h, floats(*offset), floats(*scale),
+        fileIdxStart, fileIdxEnd, debugSkipLoad, spread, floats(*overrideSize),
+        overrideTimeOff, overrideGoodRegion, loadTimeScale,
+        int(rescaleSdfValues), sdfIsoOff, repeatStartFrame), phi.ctx)
+
+
+def loadAdvectTimeSliceFromDefo(defo, dst, phi, time, blendAlpha,
+                                loadTimeScale=1., defoOffset=Vec4(0.),
+                                defoScale=Vec4(1.), defoFactor=Vec4(1.),
+                                overrideSize=Vec4(-1.), overrideTimeOff=0.,
+                                zeroVel=False, bordSkip=1, defoAniFac=1.):
+    """loadAdvectTimeSlice_OptRun (optflow4d.cpp:1951-2105) for one
+    deformation volume held in the Grid4Vec4 `defo`; dst is a DeviceArray
+    (Z, Y, X) updated in place."""
+    dz, dy, dx = dst.shape
+    check(_lib.lib().flof_advect_time_slice_opt(
+        defo.h, dst.ptr, dx, dy, dz, phi.h, time, blendAlpha, loadTimeScale,
+        floats(*defoOffset), floats(*defoScale), floats(*defoFactor),
+        floats(*overrideSize), overrideTimeOff, int(zeroVel), bordSkip,
+        defoAniFac), phi.ctx)
+
+
+def dropSdf4d(phi, seed, nspheres=1, rmin=0.15, rmax=0.15, clampCells=4.):
+    """Seeded falling-drop SDF (no reference counterpart)."""
+    check(_lib.lib().flof_drop_sdf4d(phi.h, seed, nspheres, rmin, rmax,
+                                     clampCells), phi.ctx)
+
+
+def dropFrames3d(frames, X, Y, Z, T, scale, seed, nspheres=1, rmin=0.15,
+                 rmax=0.15, clampCells=4.):
+    """The same drops as T frames at `scale` x resolution into a DeviceArray
+    (T, scale*Z, scale*Y, scale*X)."""
+    check(_lib.lib().flof_drop_frames3d(frames.ctx, frames.ptr, X, Y, Z, T,
+                                        scale, seed, nspheres, rmin, rmax,
+                                        clampCells), frames.ctx)
+
+
+def sync():
+    check(_lib.lib().flof_sync(context()), context())
