@@ -1,0 +1,260 @@
+"""Benchmark of the FlOF deformation path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-frames]
+
+A step is one device-resident multiscale FlOF solve at 64^4 (seeded drop
+SDFs, wSmooth 0.1, wEnergy 1e-4, 3 levels, every level run to relative
+signed-max residual 1e-5 -- the iteration cap is raised above the
+reference's 1000 so the residual is the same at every level).  The JSON line
+reports solves/s, the HBM roofline of the dominant kernel (k_cg_update),
+timed live with HIP events on the library's stream, and the CPU oracle timed
+on a bounded sample of the same workload on this host.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "64^4 FlOF solves/s at fixed residual; % HBM roofline; warm-start vs plain"
+HBM_PEAK = 8.0e12
+N = 64
+PARAMS = dict(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5, minGridSize=16,
+              multiStep=1, postVelBlur=0.0, doFinalProject=False,
+              resetBndWidth=-1.0, cfl=999.0)
+FIXED_RESIDUAL_CAP = 20000
+# algorithmic bytes per cell of one launch (FP32 Vec4 = 16 B per vector)
+B_UPDATE = 16 * 3 + 16       # read r,q,diag^-1; write r
+B_APPLY = 16 * 2 + 16        # read p,g; write q
+B_DIR = 16 * 4 + 16 * 2      # read x,p,r,diag^-1; write x,p
+B_ITER_MIN = 112             # each of x,r,p read+written once, g read once
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-frames", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="short workload for rocprofv3 --pmc runs: a 256 MiB "
+                         "grid copy (calibration) and 3 level-0 PCG iterations")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; every box here has one "
+              "GPU and the FlOF solve is not sharded (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from ofblend_amd import _lib
+    from ofblend_amd import manta as M
+    from ofblend_amd.abi import FLOF_KT_CG_APPLY, FLOF_KT_CG_DIR, FLOF_KT_CG_UPDATE, FLOF_KT_WARP
+
+    L = _lib.lib()
+    s = M.Solver(gridSize=M.vec3(N, N, N), fourthDim=N)
+    ctx = M.context()
+    i0 = s.create(M.Grid4Real)
+    i1 = s.create(M.Grid4Real)
+    M.dropSdf4d(i0, seed=1)
+    M.dropSdf4d(i1, seed=2)
+    vel = s.create(M.Grid4Vec4)
+    if args.pmc_workload:
+        # copy of a 64^4 Vec4 grid: 256 MiB read + 256 MiB written
+        other = s.create(M.Grid4Vec4)
+        other.copyFrom(vel)
+        M.sync()
+        M.opticalFlow4d(vel, i0, i1, wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-30,
+                        cgMaxIter=3)
+        M.sync()
+        log("bench: pmc workload done")
+        return
+
+    def solve(cap):
+        vel.clear()
+        return M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, cgMaxIter=cap, **PARAMS)
+
+    log("bench: warmup %d solve(s) at %d^4" % (args.warmup, N))
+    for _ in range(args.warmup):
+        solve(FIXED_RESIDUAL_CAP)
+    M.sync()
+    kinds = (FLOF_KT_CG_APPLY, FLOF_KT_CG_UPDATE, FLOF_KT_CG_DIR)
+    for k in kinds:
+        L.flof_ctx_timing(ctx, k, None, None)  # drop earlier records
+        L.flof_ctx_enable_timing(ctx, k, 1)
+    times = []
+    infos = []
+    for st in range(args.steps):
+        M.sync()
+        t0 = time.perf_counter()
+        err = solve(FIXED_RESIDUAL_CAP)
+        M.sync()
+        times.append(time.perf_counter() - t0)
+        infos.append(M.getLastSolveInfo().summary())
+        log("bench: step %d %.3f s iters %s" % (
+            st, times[-1], [l["iters"][0] for l in infos[-1]["levels"]]))
+    ktime = {}
+    for k in kinds:
+        n, ms = C.c_int64(), C.c_double()
+        L.flof_ctx_timing(ctx, k, C.byref(n), C.byref(ms))
+        L.flof_ctx_enable_timing(ctx, k, 0)
+        ktime[k] = (n.value, ms.value)
+    info = infos[-1]
+    levels = info["levels"]
+    iters = [l["iters"][0] for l in levels]
+    cells = [int(np.prod(l["dims"])) for l in levels]
+    cell_iters = sum(i * c for i, c in zip(iters, cells))
+    ms_step = 1000.0 * float(np.median(times))
+    solves_per_s = 1000.0 / ms_step
+
+    # roofline of the dominant kernel: algorithmic bytes of all real launches
+    # over the summed event time of every launch of that kernel
+    def kstat(kind, bpc):
+        n, ms = ktime[kind]
+        byts = bpc * cell_iters * args.steps
+        return {"launches": n, "ms_total": ms, "avg_us": 1000.0 * ms / max(1, n),
+                "GBps": byts / (ms * 1e-3) / 1e9 if ms > 0 else None}
+    kst = {"cg_update": kstat(FLOF_KT_CG_UPDATE, B_UPDATE),
+           "cg_apply": kstat(FLOF_KT_CG_APPLY, B_APPLY),
+           "cg_dir": kstat(FLOF_KT_CG_DIR, B_DIR)}
+    dom = max(kst, key=lambda k: kst[k]["ms_total"])
+    bpc = {"cg_update": B_UPDATE, "cg_apply": B_APPLY, "cg_dir": B_DIR}[dom]
+    achieved = kst[dom]["GBps"] * 1e9
+    iter_ms = sum(kst[k]["ms_total"] for k in kst) / args.steps
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                "kernel": "k_" + dom,
+                "bytes_per_cell": bpc,
+                "cells_per_launch_avg": cell_iters / max(1, sum(iters)),
+                "pcg_iteration_frac_112B": (B_ITER_MIN * cell_iters / (iter_ms * 1e-3)) / HBM_PEAK}
+
+    out = {
+        "metric": METRIC, "value": solves_per_s, "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic: seeded falling-drop SDFs (seeds 1, 2)",
+        "config": {"workload": "FlOF opticalFlowMultiscale4d 64^4, 3 levels, "
+                               "P_plain (wSmooth 0.1, wEnergy 1e-4), every level to "
+                               "relative residual 1e-5",
+                   "grid": [N, N, N, N], "levels": len(levels),
+                   "iters_per_level": iters, "final_err": info["err_final"],
+                   "residual_per_level": [l["residual"][0] for l in levels]},
+        "roofline": roofline,
+        "kernels": kst,
+    }
+
+    # the same solve with the reference's iteration cap (1000)
+    t0 = time.perf_counter()
+    solve(1000)
+    M.sync()
+    capinfo = M.getLastSolveInfo().summary()
+    out["reference_cap_1000"] = {
+        "ms": 1000.0 * (time.perf_counter() - t0),
+        "iters_per_level": [l["iters"][0] for l in capinfo["levels"]],
+        "residual_per_level": [l["residual"][0] for l in capinfo["levels"]]}
+    vcap = vel.to_numpy()
+    import hashlib
+    out["reference_cap_1000"]["sha256"] = hashlib.sha256(vcap.tobytes()).hexdigest()
+
+    if not args.no_frames:
+        out["frame_warp"] = frame_warp(M, L, ctx, s, vel, FLOF_KT_WARP)
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_iters, iters, cells, ms_step)
+        out["speedup_vs_cpu"] = out["cpu_baseline"]["s_per_solve"] / (ms_step / 1000.0)
+    print(json.dumps(out), flush=True)
+
+
+def frame_warp(M, L, ctx, s, vel, kind):
+    """Config 2's second part: 64 analytic 3D frames at 128^3 placed into a
+    128^3 x 64 level set and warped by the solved deformation (alpha 1,
+    defoFactor (2,2,2,1))."""
+    F = 2
+    fr = M.DeviceArray((N, F * N, F * N, F * N))
+    M.dropFrames3d(fr, N, N, N, N, F, seed=1)
+    sp = M.Solver(gridSize=M.vec3(F * N, F * N, F * N), fourthDim=N)
+    phi = sp.create(M.Grid4Real)
+    t0 = time.perf_counter()
+    M.loadPlaceGrid4dFrames(fr, phi, M.Vec4(0.), M.Vec4(1.))
+    M.sync()
+    t_place = time.perf_counter() - t0
+    dst = M.DeviceArray((N, F * N, F * N, F * N))
+    args = (vel.h, dst.ptr, F * N, F * N, F * N, N, 0.0, 1.0, phi.h, 1.0, 1.0,
+            _f4(0, 0, 0, 0), _f4(1, 1, 1, 1), _f4(2, 2, 2, 1), 1, 1.0)
+    _lib_check(L.flof_advect_time_slices_opt(*args), ctx)  # warm-up
+    M.sync()
+    L.flof_ctx_timing(ctx, kind, None, None)
+    L.flof_ctx_enable_timing(ctx, kind, 1)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _lib_check(L.flof_advect_time_slices_opt(*args), ctx)
+    M.sync()
+    wall = (time.perf_counter() - t0) / reps
+    n, ms = C.c_int64(), C.c_double()
+    L.flof_ctx_timing(ctx, kind, C.byref(n), C.byref(ms))
+    L.flof_ctx_enable_timing(ctx, kind, 0)
+    kern_s = ms.value / 1000.0 / max(1, n.value)
+    M3, Md = (F * N) ** 3, N ** 3
+    byts = N * (4 * M3 + 8 * M3 + 2 * 16 * Md)  # per frame: write, 2 phi slices, 2 defo slices
+    return {"frames": N, "frame": [F * N] * 3, "ms_per_64_frames": 1000 * wall,
+            "frames_per_s": N / wall, "kernel_ms": 1000 * kern_s,
+            "GBps_algorithmic": byts / kern_s / 1e9, "frac": byts / kern_s / HBM_PEAK,
+            "place_ms": 1000 * t_place}
+
+
+def _lib_check(rc, ctx):
+    from ofblend_amd._lib import check
+    check(rc, ctx)
+
+
+def _f4(*v):
+    return (C.c_float * 4)(*v)
+
+
+def cpu_baseline(k_iters, iters, cells, gpu_ms):
+    """The CPU oracle (FP32, the reference's serial CG and FP64 dot order)
+    timed on a bounded sample: the finest-level set-up plus PCG at 64^4 with
+    an iteration cap of k and 2k; the difference gives the time per iteration,
+    scaled to a full solve with the GPU's per-level iteration counts."""
+    from oracle import oracle as O
+    from ofblend_amd.abi import default_params
+    threads = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    i0 = O.drop_sdf(N, N, N, N, seed=1)
+    i1 = O.drop_sdf(N, N, N, N, seed=2)
+    v = np.zeros(i0.shape + (4,), np.float32)
+    tt = []
+    for k in (k_iters, 2 * k_iters):
+        p = default_params(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5, cgMaxIter=k)
+        t0 = time.perf_counter()
+        O.optical_flow4d(v, i0, i1, p)
+        tt.append(time.perf_counter() - t0)
+        log("bench: cpu oracle 64^4 level solve, %d iterations: %.2f s" % (k, tt[-1]))
+    t_it = (tt[1] - tt[0]) / k_iters
+    t_setup = max(0.0, tt[0] - k_iters * t_it)
+    per_cell_it = t_it / cells[0]
+    per_cell_setup = t_setup / cells[0]
+    est = sum(i * c * per_cell_it + c * per_cell_setup for i, c in zip(iters, cells))
+    return {"value": 1.0 / est, "unit": "solves/s", "cores": threads, "kind": "port",
+            "s_per_solve": est,
+            "sample": "CPU oracle (FP32-faithful, serial CG as the reference) on "
+                      "the 64^4 finest level: set-up + %d and %d PCG iterations "
+                      "(%.1f s); time per cell-iteration %.1f ns scaled to the "
+                      "solve's per-level iteration counts %s" % (
+                          k_iters, 2 * k_iters, sum(tt), per_cell_it * 1e9, iters)}
+
+
+if __name__ == "__main__":
+    main()

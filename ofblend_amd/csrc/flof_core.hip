@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -67,22 +68,6 @@ void check_launch(const char *what)
     if (e != hipSuccess)
         throw Error{FLOF_ERR_HIP,
                     std::string("launch of ") + what + ": " + hipGetErrorString(e)};
-}
-
-TimedLaunch::TimedLaunch(flof_ctx *c, int k) : ctx(c), kind(k)
-{
-    if (kind >= 0 && kind < FLOF_KT_COUNT && ctx->timers[kind].on) {
-        FLOF_HIP(hipEventCreate(&a));
-        FLOF_HIP(hipEventCreate(&b));
-        FLOF_HIP(hipEventRecord(a, ctx->stream));
-    }
-}
-TimedLaunch::~TimedLaunch()
-{
-    if (a) {
-        (void)hipEventRecord(b, ctx->stream);
-        ctx->timers[kind].pending.emplace_back(a, b);
-    }
 }
 
 void *reduce_scratch(flof_ctx *ctx, size_t bytes)
@@ -516,15 +501,12 @@ void launch_advect(flof_ctx *ctx, const float4 *vel, float vel_scale, void *grid
     Coord4 C = make_coord4(D);
     float4 vs = make_float4(vel_scale, vel_scale, vel_scale, vel_scale);
     const int use = vel_scale != 1.f;
-    TimedLaunch tl(ctx, FLOF_KT_ADVECT);
     if (vec4)
-        hipLaunchKernelGGL(k_advect<float4>, dim3(grid_blocks(ctx, n)), dim3(256), 0,
-                           ctx->stream, (float4 *)tmp, (const float4 *)grid, vel, vs,
-                           use, D, C, dt);
+        klaunch(ctx, FLOF_KT_ADVECT, k_advect<float4>, dim3(grid_blocks(ctx, n)), dim3(256),
+                (float4 *)tmp, (const float4 *)grid, vel, vs, use, D, C, dt);
     else
-        hipLaunchKernelGGL(k_advect<float>, dim3(grid_blocks(ctx, n)), dim3(256), 0,
-                           ctx->stream, (float *)tmp, (const float *)grid, vel, vs,
-                           use, D, C, dt);
+        klaunch(ctx, FLOF_KT_ADVECT, k_advect<float>, dim3(grid_blocks(ctx, n)), dim3(256),
+                (float *)tmp, (const float *)grid, vel, vs, use, D, C, dt);
     check_launch("k_advect");
 }
 
@@ -862,6 +844,16 @@ int flof_ctx_create(flof_ctx **out, int device)
     ctx->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     FLOF_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     FLOF_HIP(hipHostMalloc(&ctx->host_scratch, 4096, hipHostMallocDefault));
+    {
+        const char *ng = std::getenv("FLOF_NO_GRAPH");
+        ctx->use_graphs = !(ng && ng[0] == '1');
+        const char *bp = std::getenv("FLOF_BLOCKS_PER_CU");
+        if (bp && std::atoi(bp) > 0 && std::atoi(bp) <= 64) ctx->blocks_per_cu = std::atoi(bp);
+        const char *nt = std::getenv("FLOF_NO_TMARCH");
+        ctx->use_tmarch = !(nt && nt[0] == '1');
+        const char *tb = std::getenv("FLOF_TM_BPC");
+        if (tb && std::atoi(tb) > 0 && std::atoi(tb) <= 64) ctx->tm_blocks_per_cu = std::atoi(tb);
+    }
     *out = ctx;
     }
     catch (const flof::Error &e)

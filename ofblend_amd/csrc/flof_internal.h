@@ -7,6 +7,7 @@
 // correctly rounded sums (double-double accumulation, see dd_* below).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -70,6 +71,14 @@ struct flof_ctx {
     flof::KTimer timers[FLOF_KT_COUNT];
     // pinned host staging for scalar read-backs
     void *host_scratch = nullptr;
+    // capture CG iteration chunks as hipGraphs (FLOF_NO_GRAPH=1 disables)
+    bool use_graphs = true;
+    // workgroups per CU of the streaming kernels (FLOF_BLOCKS_PER_CU)
+    int blocks_per_cu = 8;
+    // t-marching stencil (FLOF_NO_TMARCH=1 disables) and its workgroups per
+    // CU (FLOF_TM_BPC)
+    bool use_tmarch = true;
+    int tm_blocks_per_cu = 4;
     // device scratch for reductions (partials + state), grown on demand
     void *red = nullptr;
     size_t red_bytes = 0;
@@ -121,14 +130,24 @@ struct Tmp {
     template <class T> T *as() const { return (T *)p; }
 };
 
-// timing bracket for one launch of kernel class `kind`
-struct TimedLaunch {
-    flof_ctx *ctx;
-    int kind;
-    hipEvent_t a = nullptr, b = nullptr;
-    TimedLaunch(flof_ctx *c, int k);
-    ~TimedLaunch();
-};
+// Launch on the context stream.  While timing of kernel class `kind` is on,
+// the start/stop events are attached to the dispatch packet itself
+// (hipExtLaunchKernelGGL), so they bracket the kernel's execution and not the
+// surrounding launch overhead.
+template <typename... KArgs, typename... Args>
+inline void klaunch(flof_ctx *ctx, int kind, void (*kernel)(KArgs...), dim3 grid,
+                    dim3 block, Args... args)
+{
+    if (kind >= 0 && kind < FLOF_KT_COUNT && ctx->timers[kind].on) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        FLOF_HIP(hipEventCreate(&e0));
+        FLOF_HIP(hipEventCreate(&e1));
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, e0, e1, 0, args...);
+        ctx->timers[kind].pending.emplace_back(e0, e1);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, args...);
+    }
+}
 
 // grid size for grid-stride kernels: enough waves to fill the chip
 inline int grid_blocks(flof_ctx *ctx, int64_t work, int block = 256,
@@ -142,9 +161,9 @@ inline int grid_blocks(flof_ctx *ctx, int64_t work, int block = 256,
 // same, rounded up to a multiple of the XCD count (for FOR_XCD kernels)
 #define FLOF_NXCD 8
 inline int grid_blocks_xcd(flof_ctx *ctx, int64_t work, int block = 256,
-                           int per_cu = 8)
+                           int per_cu = 0)
 {
-    int64_t g = grid_blocks(ctx, work, block, per_cu);
+    int64_t g = grid_blocks(ctx, work, block, per_cu > 0 ? per_cu : ctx->blocks_per_cu);
     g = ((g + FLOF_NXCD - 1) / FLOF_NXCD) * FLOF_NXCD;
     return (int)g;
 }

@@ -15,11 +15,13 @@
 // applied from g on the fly with the identical FP32 operation order, so
 // q = A p is bit-identical to the reference's applyMat.
 //
-// PCG per iteration (three kernels, FP64 scalars on the device):
-//   k_cg_apply : q = A p, pq = p.q                      -> alpha = sigma/pq
-//   k_cg_update: x += alpha p, r -= alpha q, max(r), z = r/diag, z.r
-//                                    -> converged? beta = z.r/sigma
-//   k_cg_dir   : p = z + beta p
+// PCG per iteration (three streaming kernels, two 1-block FP64 finalizers):
+//   k_cg_apply(_tm): q = A p, pq = p.q        -> k_cg_alpha: alpha = sigma/pq
+//   k_cg_update    : r -= alpha q, max(r), z = r/diag, z.r
+//                           -> k_cg_beta: converged? beta = z.r/sigma
+//   k_cg_dir       : x += alpha p, p = z + beta p
+// (x is updated in k_cg_dir, which reads p anyway: 208 B per cell and
+// iteration instead of 224.)
 // The host enqueues iterations in chunks and reads the device state once per
 // chunk; kernels of iterations past convergence return immediately.
 
@@ -46,7 +48,8 @@ struct CgState {
     int maxIter;
     int warm;    // warm start kept
     int zero;    // result must be zeroed (failed solve)
-    int pad[2];
+    int dir_mode;  // k_cg_dir: 0 skip, 1 x and p update, 2 x update only
+    int pad;
 };
 
 struct LevelGeom {
@@ -391,10 +394,105 @@ k_cg_apply(const float4 *__restrict__ p, const float4 *__restrict__ g,
     if (threadIdx.x == 0) part_dd[blockIdx.x] = acc;
 }
 
+// q = A p marching along t.  Work item = (3D cell c3, t segment); the thread
+// keeps p(t-1), p(t), p(t+1) in registers, so every p value is fetched once
+// from HBM; the x/y/z neighbours at time t are fetched through L2, which holds
+// the few t-planes the co-resident workgroups are working on.  Workgroups of
+// one XCD (blockIdx % 8) take a contiguous range of work items, so the
+// neighbouring rows and planes of a workgroup live in the same L2.
+struct TMarch {
+    int64_t n3;      // X*Y*Z
+    int nblk3;       // workgroups per t segment (ceil(n3 / 256))
+    int seglen;      // t steps per segment
+    int nseg;        // segments
+    int total;       // nblk3 * nseg real workgroups
+};
+
+__global__ void __launch_bounds__(256)
+k_cg_apply_tm(const float4 *__restrict__ p, const float4 *__restrict__ g,
+              float4 *__restrict__ q, LevelGeom G, TMarch M, const CgState *st,
+              DD *part_dd)
+{
+    if (st->done) return;
+    const Dims4 &D = G.D;
+    // logical workgroup id: contiguous per XCD
+    const int per = gridDim.x / FLOF_NXCD;
+    const int L = (blockIdx.x % FLOF_NXCD) * per + blockIdx.x / FLOF_NXCD;
+    DD acc{0., 0.};
+    if (L < M.total) {
+        const int seg = L / M.nblk3;
+        const int b3 = L - seg * M.nblk3;
+        const int64_t c3 = (int64_t)b3 * blockDim.x + threadIdx.x;
+        const int t0 = seg * M.seglen;
+        const int t1 = min(D.T, t0 + M.seglen);
+        if (c3 < M.n3 && t0 < t1) {
+            const int i = (int)(c3 % D.X);
+            const int j = (int)((c3 / D.X) % D.Y);
+            const int k = (int)(c3 / D.sz);
+            const bool sp_int = i >= 1 && j >= 1 && k >= 1 && i < D.X - 1 && j < D.Y - 1 &&
+                                k < D.Z - 1;
+            const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 pm = t0 > 0 ? p[c3 + D.st * (t0 - 1)] : zero4;
+            float4 pc = p[c3 + D.st * t0];
+            for (int t = t0; t < t1; ++t) {
+                const int64_t c = c3 + D.st * t;
+                const float4 pp = t + 1 < D.T ? p[c + D.st] : zero4;
+                float4 qc;
+                if (sp_int && t >= 1 && t < D.T - 1) {
+                    const float4 gc = g[c];
+                    float4 v = zero4;
+                    if (G.use_offd) {
+                        const float4 nb[8] = {pm,
+                                              p[c - D.sz],
+                                              p[c - D.X],
+                                              p[c - 1],
+                                              p[c + 1],
+                                              p[c + D.X],
+                                              p[c + D.sz],
+                                              pp};
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) {
+                            v.x = v.x + G.offd * nb[m].x;
+                            v.y = v.y + G.offd * nb[m].y;
+                            v.z = v.z + G.offd * nb[m].z;
+                            v.w = v.w + G.offd * nb[m].w;
+                        }
+                    }
+                    const float gg[4] = {gc.x, gc.y, gc.z, gc.w};
+                    const float pv[4] = {pc.x, pc.y, pc.z, pc.w};
+                    float o[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+#pragma unroll
+                        for (int m = 0; m < 4; ++m) {
+                            float b = gg[d] * gg[m];
+                            if (d == m) b = b + G.diag;
+                            o[d] = o[d] + b * pv[m];
+                        }
+                    }
+                    qc = make_float4(o[0], o[1], o[2], o[3]);
+                } else {
+                    qc = apply_I(pc);
+                }
+                q[c] = qc;
+                dd_add4(acc, pc, qc);
+                pm = pc;
+                pc = pp;
+            }
+        }
+    }
+    acc = block_dd(acc);
+    if (threadIdx.x == 0) part_dd[blockIdx.x] = acc;
+}
+
 // alpha = sigma / p.q
 __global__ void k_cg_alpha(CgState *st, const DD *pd, int nb)
 {
-    if (st->done) return;
+    if (st->done) {
+        // iterations after the last one: the direction kernel does nothing
+        if (threadIdx.x == 0) st->dir_mode = 0;
+        return;
+    }
     DD s;
     float m;
     fin_dd_max(pd, nullptr, nb, s, m);
@@ -402,28 +500,24 @@ __global__ void k_cg_alpha(CgState *st, const DD *pd, int nb)
 }
 
 __global__ void __launch_bounds__(256)
-k_cg_update(float4 *__restrict__ x, float4 *__restrict__ r,
-            const float4 *__restrict__ p, const float4 *__restrict__ q,
+k_cg_update(float4 *__restrict__ r, const float4 *__restrict__ q,
             const float4 *__restrict__ prec, int64_t n, const CgState *st, DD *part_dd,
             float *part_max)
 {
+    // r -= alpha q (optflow4d.cpp:310); x += alpha p is applied by k_cg_dir,
+    // which reads p anyway, with the same operands
     if (st->done) return;
-    const double alpha = st->alpha, nalpha = -alpha;
+    const double nalpha = -st->alpha;
     DD sig{0., 0.};
     float mx = -INFINITY;
     FOR_XCD(c, n)
     {
-        const float4 pc = p[c], qc = q[c];
-        float4 xc = x[c], rc = r[c];
-        xc.x = (float)((double)xc.x + alpha * (double)pc.x);
-        xc.y = (float)((double)xc.y + alpha * (double)pc.y);
-        xc.z = (float)((double)xc.z + alpha * (double)pc.z);
-        xc.w = (float)((double)xc.w + alpha * (double)pc.w);
+        const float4 qc = q[c];
+        float4 rc = r[c];
         rc.x = (float)((double)rc.x + nalpha * (double)qc.x);
         rc.y = (float)((double)rc.y + nalpha * (double)qc.y);
         rc.z = (float)((double)rc.z + nalpha * (double)qc.z);
         rc.w = (float)((double)rc.w + nalpha * (double)qc.w);
-        x[c] = xc;
         r[c] = rc;
         const float4 z = f4mul(rc, prec[c]);
         mx = fmaxf(mx, fmaxf(fmaxf(rc.x, rc.y), fmaxf(rc.z, rc.w)));
@@ -438,7 +532,7 @@ k_cg_update(float4 *__restrict__ x, float4 *__restrict__ r,
 }
 
 // signed-max stopping test, then beta = z.r / sigma (optflow4d.cpp:311-325)
-__global__ void k_cg_beta(CgState *st, const DD *pd, const float *pm, int nb, int it)
+__global__ void k_cg_beta(CgState *st, const DD *pd, const float *pm, int nb)
 {
     if (st->done) return;
     DD s;
@@ -447,42 +541,59 @@ __global__ void k_cg_beta(CgState *st, const DD *pd, const float *pm, int nb, in
     if (threadIdx.x == 0) {
         const double residual = (double)m;
         const double sigmaNew = s.hi + s.lo;
+        const int it = st->iters;  // iterations completed before this one
         st->iters = it + 1;
         st->resRel = (double)(float)(residual / st->resIni);
         if (residual <= st->acc) {
             st->status = 0;
             st->done = 1;
+            st->dir_mode = 2;  // the last x update only
         } else if (sigmaNew != sigmaNew || residual != residual) {
             // a NaN residual ends with a zero result (optflow4d.cpp:509-514)
             st->status = 2;
             st->zero = 1;
             st->done = 1;
+            st->dir_mode = 0;
         } else {
             st->beta = sigmaNew / st->sigma;
             st->sigma = sigmaNew;
+            st->dir_mode = 1;
             if (it + 1 >= st->maxIter) {
                 st->status = 1;
                 st->done = 1;
+                st->dir_mode = 2;
             }
         }
     }
 }
 
 __global__ void __launch_bounds__(256)
-k_cg_dir(float4 *__restrict__ p, const float4 *__restrict__ r,
+k_cg_dir(float4 *__restrict__ x, float4 *__restrict__ p, const float4 *__restrict__ r,
          const float4 *__restrict__ prec, int64_t n, const CgState *st)
 {
-    if (st->done) return;
-    const double beta = st->beta;
+    // x += alpha p (optflow4d.cpp:309), then p = z + beta p (:318-323) unless
+    // this was the last iteration
+    const int mode = st->dir_mode;
+    if (mode == 0) return;
+    const double alpha = st->alpha, beta = st->beta;
     FOR_XCD(c, n)
     {
-        const float4 z = f4mul(r[c], prec[c]);
-        float4 pc = p[c];
-        pc.x = (float)((double)z.x + beta * (double)pc.x);
-        pc.y = (float)((double)z.y + beta * (double)pc.y);
-        pc.z = (float)((double)z.z + beta * (double)pc.z);
-        pc.w = (float)((double)z.w + beta * (double)pc.w);
-        p[c] = pc;
+        const float4 pc = p[c];
+        float4 xc = x[c];
+        xc.x = (float)((double)xc.x + alpha * (double)pc.x);
+        xc.y = (float)((double)xc.y + alpha * (double)pc.y);
+        xc.z = (float)((double)xc.z + alpha * (double)pc.z);
+        xc.w = (float)((double)xc.w + alpha * (double)pc.w);
+        x[c] = xc;
+        if (mode == 1) {
+            const float4 z = f4mul(r[c], prec[c]);
+            float4 np;
+            np.x = (float)((double)z.x + beta * (double)pc.x);
+            np.y = (float)((double)z.y + beta * (double)pc.y);
+            np.z = (float)((double)z.z + beta * (double)pc.z);
+            np.w = (float)((double)z.w + beta * (double)pc.w);
+            p[c] = np;
+        }
     }
 }
 
@@ -743,11 +854,32 @@ struct SolveOut {
     double resRel = 0.;
 };
 
+// t-march decomposition: enough workgroups to fill the chip, t split into
+// segments when one t-slab has too few cells
+static TMarch make_tmarch(flof_ctx *ctx, const Dims4 &D)
+{
+    TMarch M;
+    M.n3 = D.st;
+    M.nblk3 = (int)((M.n3 + 255) / 256);
+    const int target = ctx->num_cu * ctx->tm_blocks_per_cu;
+    int nseg = (target + M.nblk3 - 1) / M.nblk3;
+    if (nseg < 1) nseg = 1;
+    if (nseg > D.T) nseg = D.T;
+    M.seglen = (D.T + nseg - 1) / nseg;
+    M.nseg = (D.T + M.seglen - 1) / M.seglen;
+    M.total = M.nblk3 * M.nseg;
+    return M;
+}
+
 struct LevelWork {
     Tmp g, prec, x, r, p, q, st, part;
-    int nb = 0;
-    LevelWork(flof_ctx *ctx, int64_t n)
+    int nb = 0;     // grid of the streaming kernels
+    int nb_tm = 0;  // grid of the t-marching apply
+    int cap = 0;    // partial slots
+    TMarch tm;
+    LevelWork(flof_ctx *ctx, const Dims4 &D)
     {
+        const int64_t n = (int64_t)D.st * D.T;
         const size_t b = (size_t)n * 16;
         g = Tmp(ctx, b);
         prec = Tmp(ctx, b);
@@ -756,13 +888,16 @@ struct LevelWork {
         p = Tmp(ctx, b);
         q = Tmp(ctx, b);
         nb = grid_blocks_xcd(ctx, n);
+        tm = make_tmarch(ctx, D);
+        nb_tm = ((tm.total + FLOF_NXCD - 1) / FLOF_NXCD) * FLOF_NXCD;
+        cap = std::max(nb, nb_tm);
         st = Tmp(ctx, sizeof(CgState));
-        part = Tmp(ctx, (size_t)nb * (2 * sizeof(DD) + sizeof(float)) + 256);
+        part = Tmp(ctx, (size_t)cap * (2 * sizeof(DD) + sizeof(float)) + 256);
         FLOF_HIP(hipMemsetAsync(st.p, 0, sizeof(CgState), ctx->stream));
     }
     DD *pdd() const { return part.as<DD>(); }
-    DD *pdd2() const { return part.as<DD>() + nb; }
-    float *pmax() const { return (float *)(part.as<DD>() + 2 * nb); }
+    DD *pdd2() const { return part.as<DD>() + cap; }
+    float *pmax() const { return (float *)(part.as<DD>() + 2 * cap); }
 };
 
 // Solves one level.  vin: incoming flow for the rhs (may be null = zero).
@@ -776,7 +911,7 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
                  "opticalFlow4d: bad size");
     const int64_t n = (int64_t)D.st * D.T;
     FLOF_REQUIRE(n < (1ll << 31), FLOF_ERR_DIM, "opticalFlow4d: grid too large");
-    LevelWork W(ctx, n);
+    LevelWork W(ctx, D);
     CgState *st = W.st.as<CgState>();
     const float mDx = (float)(1. / D.X);
     const float dxf = (float)(1. / (2. * (double)mDx));
@@ -828,45 +963,96 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
                            W.pdd(), W.pmax(), nb);
         check_launch("k_warm_adopt_fin");
     }
+    // Iterations are enqueued in chunks of CHUNK (as one captured hipGraph
+    // unless per-kernel timing is on); after each chunk the state is copied to
+    // pinned host memory behind an event, and the host waits only for the
+    // chunk before the last one, so the GPU never idles on the check.
+    // Kernels of iterations after convergence (or the cap, counted on the
+    // device) return at once.
+    const int CHUNK = 16;
+    bool timed = false;
+    for (int k : {FLOF_KT_CG_APPLY, FLOF_KT_CG_UPDATE, FLOF_KT_CG_DIR})
+        timed = timed || ctx->timers[k].on;
+    auto enqueue_iters = [&](int count, bool with_timers) {
+        const int ka = with_timers ? FLOF_KT_CG_APPLY : -1;
+        const int ku = with_timers ? FLOF_KT_CG_UPDATE : -1;
+        const int kd = with_timers ? FLOF_KT_CG_DIR : -1;
+        for (int k = 0; k < count; ++k) {
+            if (ctx->use_tmarch) {
+                klaunch(ctx, ka, k_cg_apply_tm, dim3(W.nb_tm), dim3(256),
+                        (const float4 *)W.p.as<float4>(), (const float4 *)W.g.as<float4>(),
+                        W.q.as<float4>(), G, W.tm, (const CgState *)st, W.pdd());
+                hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st,
+                                   W.pdd(), W.nb_tm);
+            } else {
+                klaunch(ctx, ka, k_cg_apply, dim3(nb), dim3(256),
+                        (const float4 *)W.p.as<float4>(), (const float4 *)W.g.as<float4>(),
+                        W.q.as<float4>(), G, (const CgState *)st, W.pdd());
+                hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st,
+                                   W.pdd(), nb);
+            }
+            klaunch(ctx, ku, k_cg_update, dim3(nb), dim3(256), W.r.as<float4>(),
+                    (const float4 *)W.q.as<float4>(), (const float4 *)W.prec.as<float4>(), n,
+                    (const CgState *)st, W.pdd(), W.pmax());
+            hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
+                               W.pmax(), nb);
+            klaunch(ctx, kd, k_cg_dir, dim3(nb), dim3(256), W.x.as<float4>(), W.p.as<float4>(),
+                    (const float4 *)W.r.as<float4>(), (const float4 *)W.prec.as<float4>(), n,
+                    (const CgState *)st);
+        }
+    };
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    if (!timed && ctx->use_graphs) {
+        FLOF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_iters(CHUNK, false);
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        if (ce == hipSuccess && graph)
+            ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ce != hipSuccess) {
+            (void)hipGetLastError();
+            if (graph) (void)hipGraphDestroy(graph);
+            graph = nullptr;
+            exec = nullptr;
+        }
+    }
+    CgState *slots = (CgState *)ctx->host_scratch;  // pinned, 2 slots
+    hipEvent_t ev[2];
+    FLOF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    FLOF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     CgState hs;
-    int launched = 0, chunk = 4;
     FLOF_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
     FLOF_HIP(hipStreamSynchronize(ctx->stream));
-    while (!hs.done && launched < maxIter) {
-        const int m = std::min(chunk, maxIter - launched);
-        for (int k = 0; k < m; ++k) {
-            const int it = launched + k;
-            {
-                TimedLaunch tl(ctx, FLOF_KT_CG_APPLY);
-                hipLaunchKernelGGL(k_cg_apply, dim3(nb), dim3(256), 0, ctx->stream,
-                                   W.p.as<float4>(), W.g.as<float4>(), W.q.as<float4>(), G,
-                                   st, W.pdd());
-            }
-            hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
-                               nb);
-            {
-                TimedLaunch tl(ctx, FLOF_KT_CG_UPDATE);
-                hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, ctx->stream,
-                                   W.x.as<float4>(), W.r.as<float4>(), W.p.as<float4>(),
-                                   W.q.as<float4>(), W.prec.as<float4>(), n, st, W.pdd(),
-                                   W.pmax());
-            }
-            hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
-                               W.pmax(), nb, it);
-            {
-                TimedLaunch tl(ctx, FLOF_KT_CG_DIR);
-                hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(256), 0, ctx->stream,
-                                   W.p.as<float4>(), W.r.as<float4>(), W.prec.as<float4>(),
-                                   n, st);
-            }
+    int64_t chunks = 0;
+    const int64_t max_chunks = (int64_t)maxIter / CHUNK + 3;
+    hipError_t loop_err = hipSuccess;
+    while (!hs.done && chunks < max_chunks) {
+        if (exec) {
+            loop_err = hipGraphLaunch(exec, ctx->stream);
+            if (loop_err != hipSuccess) break;
+        } else {
+            enqueue_iters(CHUNK, timed);
         }
-        check_launch("k_cg_*");
-        launched += m;
-        chunk = std::min(chunk * 2, 64);
-        FLOF_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost,
-                                ctx->stream));
-        FLOF_HIP(hipStreamSynchronize(ctx->stream));
+        loop_err = hipMemcpyAsync(&slots[chunks & 1], st, sizeof(CgState),
+                                  hipMemcpyDeviceToHost, ctx->stream);
+        if (loop_err == hipSuccess) loop_err = hipEventRecord(ev[chunks & 1], ctx->stream);
+        if (loop_err != hipSuccess) break;
+        ++chunks;
+        if (chunks >= 2) {
+            // state after the previous chunk
+            loop_err = hipEventSynchronize(ev[chunks & 1]);
+            if (loop_err != hipSuccess) break;
+            hs = slots[chunks & 1];
+        }
     }
+    if (loop_err == hipSuccess) loop_err = hipStreamSynchronize(ctx->stream);
+    if (loop_err == hipSuccess && chunks >= 1) hs = slots[(chunks - 1) & 1];
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    FLOF_HIP(loop_err);
+    check_launch("k_cg_*");
     hipLaunchKernelGGL(k_of_finish, dim3(W.nb), dim3(256), 0, ctx->stream, W.x.as<float4>(),
                        out, n, mDx, accumulate ? 1 : 0, st);
     check_launch("k_of_finish");

@@ -280,11 +280,12 @@ def opticalFlowMultiscale4d(vel, i0, i1, rhsT=None, wSmooth=0., wEnergy=0.,
 
 def opticalFlow4d(vel, i0, i1, rhsT=None, wSmooth=0., wEnergy=0.,
                   postVelBlur=0., cgAccuracy=1e-04, blurType=1,
-                  resetBndWidth=-1.):
-    """optflow4d.cpp:2110-2119."""
+                  resetBndWidth=-1., cgMaxIter=1000):
+    """optflow4d.cpp:2110-2119 (cgMaxIter: the reference's fixed cap of 1000,
+    optflow4d.cpp:502, exposed as an extension)."""
     p = _params(wSmooth=wSmooth, wEnergy=wEnergy, postVelBlur=postVelBlur,
                 cgAccuracy=cgAccuracy, blurType=blurType,
-                resetBndWidth=resetBndWidth)
+                resetBndWidth=resetBndWidth, cgMaxIter=cgMaxIter)
     check(_lib.lib().flof_optical_flow4d(
         vel.h, i0.h, i1.h, rhsT.h if rhsT is not None else None, C.byref(p),
         C.byref(_LAST_INFO)), vel.ctx)

@@ -203,26 +203,41 @@ def test_multiscale_small_bit_exact():
     assert err == np.float32(rerr)
 
 
-def test_multiscale_c1_parity():
-    """Config 1: 32^4, 3 levels, P_plain.  Bit-exact against the oracle with
-    the GPU's summation; within the reference's own FP32 summation-order noise
-    of the serial-sum oracle, and reported against FP64."""
-    v, err, info, (i0, i1, p) = _multiscale_case(32, 32, 8)
-    with O.sum_mode(1):
-        ref1, _, rinfo = O.multiscale4d(np.zeros(v.shape, np.float32), i0, i1, p)
-    say("iters gpu", [l["iters"] for l in info["levels"]],
-        "oracle", [l["iters"] for l in rinfo.summary()["levels"]])
-    d1 = np.abs(v - ref1).max()
-    say("C1 max|gpu - oracle(exact sums)| =", d1)
-    assert d1 == 0.0
-    ref0, _, _ = O.multiscale4d(np.zeros(v.shape, np.float32), i0, i1, p)
-    ref64, _, _ = O.multiscale4d(np.zeros(v.shape, np.float64),
-                                 i0.astype(np.float64), i1.astype(np.float64), p)
-    d0 = np.abs(v - ref0).max()
-    d64 = np.abs(v.astype(np.float64) - ref64).max()
-    say("C1 max|gpu - oracle(serial sums)| =", d0, " max|gpu - FP64| =", d64,
-        " max|oracle32 - FP64| =", np.abs(ref0.astype(np.float64) - ref64).max())
-    assert d0 <= 2e-3 and d64 <= 2e-3
+def _digests():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "digests.json")
+    return json.load(open(path)) if os.path.exists(path) else {}
+
+
+@pytest.mark.parametrize("case", ["c1", "c2"])
+def test_multiscale_golden_digest(case):
+    """Configs 1 (32^4, 3 levels) and 2 (64^4, 3 levels), P_plain: the GPU
+    flow field is bit-identical to the oracle's (correctly rounded sums),
+    whose SHA-256 tests/golden/make_digests.py recorded; the digest file also
+    records the serial-sum oracle's distance to it."""
+    import hashlib
+    d = _digests().get(case)
+    if d is None:
+        pytest.skip("no golden digest for %s" % case)
+    c = d["config"]
+    N = c["N"]
+    say("case", case, "N", N)
+    i0, i1 = _drops(N, N)
+    s = solver(N, N, N, N)
+    vel = s.create(M.Grid4Vec4)
+    err = M.opticalFlowMultiscale4d(vel=vel, i0=real(s, i0), i1=real(s, i1),
+                                    minGridSize=c["minGridSize"],
+                                    cgMaxIter=c["cgMaxIter"], **O.P_PLAIN)
+    info = M.getLastSolveInfo().summary()
+    v = vel.to_numpy()
+    iters = [l["iters"][0] for l in info["levels"]]
+    say(case, "iters gpu", iters, "oracle", d["iters"], "err", err, d["err"],
+        "serial-sum oracle distance", d["serial_sum_oracle"])
+    assert iters == d["iters"]
+    assert hashlib.sha256(v.tobytes()).hexdigest() == d["sha256"]
+    assert err == np.float32(d["err"])
 
 
 def test_frames_place_and_warp_bit_exact():
@@ -278,3 +293,43 @@ def test_multiscale_flof_params_close():
     say("P_flof 24^4: max diff", d.max(), "rms", np.sqrt((d ** 2).mean()),
         "err gpu", err, "oracle", rerr)
     assert np.sqrt((d ** 2).mean()) <= 1e-2
+
+
+def test_warm_start_bit_exact_and_fallback():
+    """Warm start (extension): the guess (warm - incoming flow) seeds every
+    level's CG; kept only when ||b - A x0|| <= ||b||.  Bit-exact against the
+    oracle with the same guess; a useless guess falls back to the cold start
+    (identical to the plain solve)."""
+    N, T, mg = 16, 16, 4
+    i0, i1 = _drops(N, T)
+    s = solver(N, N, N, T)
+    g0, g1 = real(s, i0), real(s, i1)
+    prm = dict(O.P_PLAIN, minGridSize=mg)
+    cold = s.create(M.Grid4Vec4)
+    M.opticalFlowMultiscale4d(vel=cold, i0=g0, i1=g1, **prm)
+    cold_info = M.getLastSolveInfo().summary()
+    vcold = cold.to_numpy()
+    # a perturbed converged flow as the guess
+    guess = (vcold + RNG.standard_normal(vcold.shape).astype(np.float32) * 0.05)
+    w = vec4(s, guess)
+    vel = s.create(M.Grid4Vec4)
+    M.opticalFlowMultiscale4d(vel=vel, i0=g0, i1=g1, warm=w, **prm)
+    info = M.getLastSolveInfo().summary()
+    with O.sum_mode(1):
+        ref, _, rinfo = O.multiscale4d(np.zeros_like(vcold), i0, i1,
+                                       default_params(**prm), warm=guess)
+    r = rinfo.summary()
+    say("warm iters", [l["iters"] for l in info["levels"]], "oracle",
+        [l["iters"] for l in r["levels"]], "cold", [l["iters"] for l in cold_info["levels"]],
+        "warm used", [l["warm_used"] for l in info["levels"]])
+    assert [l["iters"] for l in info["levels"]] == [l["iters"] for l in r["levels"]]
+    assert [l["warm_used"] for l in info["levels"]] == [l["warm_used"] for l in r["levels"]]
+    assert np.array_equal(vel.to_numpy(), ref)
+    assert info["levels"][0]["iters"][0] < cold_info["levels"][0]["iters"][0]
+    # a wild guess is rejected: same as the cold solve
+    bad = vec4(s, (RNG.standard_normal(vcold.shape) * 1e3).astype(np.float32))
+    vb = s.create(M.Grid4Vec4)
+    M.opticalFlowMultiscale4d(vel=vb, i0=g0, i1=g1, warm=bad, **prm)
+    binfo = M.getLastSolveInfo().summary()
+    assert [l["warm_used"] for l in binfo["levels"]] == [0] * len(binfo["levels"])
+    assert np.array_equal(vb.to_numpy(), vcold)
