@@ -54,6 +54,7 @@ enum flof_dtype {
 
 typedef struct flof_ctx flof_ctx;
 typedef struct flof_grid4d flof_grid4d;
+typedef struct flof_warp_state flof_warp_state;
 
 /* Parameters of opticalFlowMultiscale4d / opticalFlow4d.  Field names and
  * defaults follow the keyword arguments of
@@ -257,6 +258,54 @@ int flof_advect_time_slices_opt(flof_grid4d *defo, float *dst_dev, int dx,
                                 const float defoScale[4],
                                 const float defoFactor[4], int bordSkip,
                                 float defoAniFac);
+
+/* ---- input preprocessing (optflow4d.cpp:1254-1266,1361-1406,1598-1616) -- */
+/* repeatFrame4d(phi, srct, range=0, bnd=0). */
+int flof_repeat_frame4d(flof_grid4d *phi, float srct, float range, int bnd);
+/* extrap4dLsSimple(phi, distance=4, inside=false). */
+int flof_extrap4d_ls_simple(flof_grid4d *phi, int distance, int inside);
+/* shiftForwGrid4d(phi, overrideGoodRegion=0). */
+int flof_shift_forw_grid4d(flof_grid4d *phi, int overrideGoodRegion);
+
+/* ---- .uni files (fileio.cpp:38-44,572-599,655-731,834-999) ------------- */
+/* getUniFileSize: X, Y, Z of an MNT2 (3D) or M4T2 (4D) file, T (1 for 3D). */
+int flof_uni_file_size(const char *name, int dims[4]);
+/* writeGrid4dUni from host memory (M4T2, gzip level 1). */
+int flof_uni_write4d(const char *name, const void *host, int X, int Y, int Z,
+                     int T, int elem);
+/* readGridUni / readGrid4dUni into host memory; dims and element type must
+ * match the file (T = 1 for a 3D file). */
+int flof_uni_read(const char *name, void *host, int X, int Y, int Z, int T,
+                  int elem);
+/* Grid4d::save / Grid4d::load (grid4d.cpp:122-138). */
+int flof_grid4d_save(flof_grid4d *g, const char *name);
+int flof_grid4d_load(flof_grid4d *g, const char *name);
+
+/* ---- file-based frame functions (optflow4d.cpp:1464-1595,1871-2105) ----- */
+/* loadPlaceGrid4d(fname, ...): `fname` is the reference's snprintf pattern
+ * with one integer conversion; frames are read from MNT2 .uni files. */
+int flof_load_place_grid4d_uni(const char *fname, flof_grid4d *phi,
+                               const float offset[4], const float scale[4],
+                               int fileIdxStart, int fileIdxEnd,
+                               int debugSkipLoad, float spread,
+                               const float overrideSize[4],
+                               float overrideTimeOff, int overrideGoodRegion,
+                               float loadTimeScale, int rescaleSdfValues,
+                               float sdfIsoOff, float repeatStartFrame);
+/* loadAdvectTimeSlice_OptInit (one deformation volume, useDefoVols off):
+ * loads the M4T2 deformation into device memory; the handle replaces the
+ * reference's int-keyed global map. */
+int flof_warp_state_open(flof_ctx *ctx, const char *fname,
+                         flof_warp_state **out);
+/* loadAdvectTimeSlice_OptRun with the state's deformation. */
+int flof_warp_state_run(flof_warp_state *s, float *dst_dev, int dx, int dy,
+                        int dz, flof_grid4d *phi, float time, float blendAlpha,
+                        float loadTimeScale, const float defoOffset[4],
+                        const float defoScale[4], const float defoFactor[4],
+                        const float overrideSize[4], float overrideTimeOff,
+                        int zeroVel, int bordSkip, float defoAniFac);
+/* loadAdvectTimeSlice_Finish. */
+int flof_warp_state_close(flof_warp_state *s);
 
 /* ---- synthetic inputs (no reference counterpart) ----------------------- */
 /* Seeded falling-drop SDF (replaces the FLIP input simulation of

@@ -89,6 +89,20 @@ SIGNATURES = {
     "flof_drop_sdf4d": ([_P, _U64, _I, _D, _D, _D], _I),
     "flof_drop_frames3d": ([_P, _P, _I, _I, _I, _I, _I, _U64, _I, _D, _D, _D],
                            _I),
+    "flof_repeat_frame4d": ([_P, _F, _F, _I], _I),
+    "flof_extrap4d_ls_simple": ([_P, _I, _I], _I),
+    "flof_shift_forw_grid4d": ([_P, _I], _I),
+    "flof_uni_file_size": ([C.c_char_p, C.POINTER(C.c_int)], _I),
+    "flof_uni_write4d": ([C.c_char_p, _P, _I, _I, _I, _I, _I], _I),
+    "flof_uni_read": ([C.c_char_p, _P, _I, _I, _I, _I, _I], _I),
+    "flof_grid4d_save": ([_P, C.c_char_p], _I),
+    "flof_grid4d_load": ([_P, C.c_char_p], _I),
+    "flof_load_place_grid4d_uni": ([C.c_char_p, _P, _FP, _FP, _I, _I, _I, _F, _FP,
+                                    _F, _I, _F, _I, _F, _F], _I),
+    "flof_warp_state_open": ([_P, C.c_char_p, C.POINTER(_P)], _I),
+    "flof_warp_state_run": ([_P, _P, _I, _I, _I, _P, _F, _F, _F, _FP, _FP, _FP, _FP,
+                             _F, _I, _I, _F], _I),
+    "flof_warp_state_close": ([_P], _I),
     "flof_dev_alloc": ([_P, _SZ, C.POINTER(_P)], _I),
     "flof_dev_free": ([_P, _P], _I),
     "flof_dev_upload": ([_P, _P, _P, _SZ], _I),

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build", "flof_hip")
 LIB = os.path.join(HERE, "libflof_hip.so")
-SOURCES = ["flof_core.hip", "flof_solve.hip", "flof_frames.hip"]
+SOURCES = ["flof_core.hip", "flof_solve.hip", "flof_frames.hip", "flof_prep.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -56,7 +56,7 @@ def build(verbose=False, jobs=None):
             if o:
                 print(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"])
     os.replace(tmp, LIB)
     return LIB
 

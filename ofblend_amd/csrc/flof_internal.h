@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -82,6 +83,11 @@ struct flof_ctx {
     // device scratch for reductions (partials + state), grown on demand
     void *red = nullptr;
     size_t red_bytes = 0;
+};
+
+struct flof_warp_state {
+    flof_ctx *ctx;
+    flof_grid4d *defo;  // the deformation, device resident
 };
 
 struct flof_grid4d {
@@ -465,6 +471,12 @@ void check_launch(const char *what);
 float host_max_norm_vec4(flof_ctx *ctx, const float4 *v, int64_t n);
 double host_ls_diff(flof_ctx *ctx, const float *a, const float *b, float *out,
                     const Dims4 &D, float corr, int bnd);
+void place_frames(flof_grid4d *phi, int sx, int sy, int sz,
+                  const std::function<const float *(int)> &frame_of, const float offset[4],
+                  const float scale[4], int fileIdxStart, int fileIdxEnd, int debugSkipLoad,
+                  float spread, const float overrideSize[4], float overrideTimeOff,
+                  int overrideGoodRegion, float loadTimeScale, int rescaleSdfValues,
+                  float sdfIsoOff, float repeatStartFrame);
 void launch_resample(flof_ctx *ctx, void *dst, const Dims4 &Dd, const void *src,
                      const Dims4 &Ds, bool vec4, const float fac[4],
                      const float off[4]);

@@ -218,6 +218,13 @@ class Grid4dBase:
         check(_lib.lib().flof_grid4d_set_bound_neumann(self.h, boundaryWidth),
               self.ctx)
 
+    # .uni files (grid4d.cpp:122-138, fileio.cpp:834-999)
+    def save(self, name):
+        check(_lib.lib().flof_grid4d_save(self.h, str(name).encode()), self.ctx)
+
+    def load(self, name):
+        check(_lib.lib().flof_grid4d_load(self.h, str(name).encode()), self.ctx)
+
 
 class Grid4Real(Grid4dBase):
     ELEM = FLOF_REAL
@@ -434,6 +441,108 @@ def loadAdvectTimeSliceFromDefo(defo, dst, phi, time, blendAlpha,
         floats(*defoOffset), floats(*defoScale), floats(*defoFactor),
         floats(*overrideSize), overrideTimeOff, int(zeroVel), bordSkip,
         defoAniFac), phi.ctx)
+
+
+def repeatFrame4d(phi, srct, range=0., bnd=0):
+    """optflow4d.cpp:1254-1266."""
+    check(_lib.lib().flof_repeat_frame4d(phi.h, srct, range, bnd), phi.ctx)
+
+
+def extrap4dLsSimple(phi, distance=4, inside=False):
+    """optflow4d.cpp:1361-1406."""
+    check(_lib.lib().flof_extrap4d_ls_simple(phi.h, distance, int(inside)),
+          phi.ctx)
+
+
+def shiftForwGrid4d(phi, overrideGoodRegion=0):
+    """optflow4d.cpp:1598-1616."""
+    check(_lib.lib().flof_shift_forw_grid4d(phi.h, overrideGoodRegion), phi.ctx)
+
+
+def getUniFileSize(name):
+    """fileio.cpp:572-599: (x, y, z, t) of a .uni file (t = 1 for 3D)."""
+    d = (C.c_int * 4)()
+    rc = _lib.lib().flof_uni_file_size(str(name).encode(), d)
+    if rc != 0:
+        raise FlofError(rc, "can't read header of %s" % name)
+    return tuple(d)
+
+
+def readUni(name, shape4, elem=FLOF_REAL):
+    """A .uni file into a host array ((T,Z,Y,X) or (T,Z,Y,X,4))."""
+    X, Y, Z, T = shape4
+    shape = (T, Z, Y, X) + ((4,) if elem == FLOF_VEC4 else ())
+    a = np.empty(shape, dtype=np.int32 if elem == FLOF_INT else np.float32)
+    rc = _lib.lib().flof_uni_read(str(name).encode(), a.ctypes.data_as(C.c_void_p),
+                                  X, Y, Z, T, elem)
+    if rc != 0:
+        raise FlofError(rc, "can't read %s" % name)
+    return a
+
+
+def writeUni4d(name, a, elem=None):
+    """A host array (T,Z,Y,X[,4]) as an M4T2 .uni file."""
+    a = np.ascontiguousarray(a)
+    if elem is None:
+        elem = FLOF_VEC4 if a.ndim == 5 else (FLOF_INT if a.dtype == np.int32 else FLOF_REAL)
+    a = np.ascontiguousarray(a, dtype=np.int32 if elem == FLOF_INT else np.float32)
+    T, Z, Y, X = a.shape[:4]
+    rc = _lib.lib().flof_uni_write4d(str(name).encode(), a.ctypes.data_as(C.c_void_p),
+                                     X, Y, Z, T, elem)
+    if rc != 0:
+        raise FlofError(rc, "can't write %s" % name)
+
+
+def loadPlaceGrid4d(fname, phi, offset, scale, fileIdxStart=-1, fileIdxEnd=-1,
+                    debugSkipLoad=999999, spread=1., overrideSize=Vec4(-1.),
+                    overrideTimeOff=0., overrideGoodRegion=0, loadTimeScale=1.,
+                    rescaleSdfValues=False, sdfIsoOff=0., repeatStartFrame=0.):
+    """optflow4d.cpp:1464-1595: frames from the .uni files `fname % index`."""
+    check(_lib.lib().flof_load_place_grid4d_uni(
+        str(fname).encode(), phi.h, floats(*offset), floats(*scale), fileIdxStart,
+        fileIdxEnd, debugSkipLoad, spread, floats(*overrideSize), overrideTimeOff,
+        overrideGoodRegion, loadTimeScale, int(rescaleSdfValues), sdfIsoOff,
+        repeatStartFrame), phi.ctx)
+
+
+_LATS = {}
+
+
+def loadAdvectTimeSlice_OptInit(ID, fname1, useDefoVols=False, doAligned=False,
+                                partialLoadFac=0.2):
+    """optflow4d.cpp:1871-1913 for one deformation volume (useDefoVols off);
+    the deformation is loaded to the GPU once."""
+    if useDefoVols:
+        raise FlofError(-1, "deformation volume blending is not available; "
+                            "use useDefoVols=False")
+    loadAdvectTimeSlice_Finish(ID)
+    h = C.c_void_p()
+    ctx = context()
+    check(_lib.lib().flof_warp_state_open(ctx, str(fname1).encode(), C.byref(h)), ctx)
+    _LATS[ID] = h
+
+
+def loadAdvectTimeSlice_OptRun(ID, fname, dst, phi, time, blendAlpha, loadTimeScale,
+                               defoOffset, defoScale, defoFactor, overrideSize=Vec4(-1.),
+                               overrideTimeOff=0., debugVel=None, debugVelT=None,
+                               zeroVel=False, thirdAlpha=0., bordSkip=1, fourthAlpha=0.,
+                               defoAniFac=1.):
+    """optflow4d.cpp:1951-2105 (dst: DeviceArray (Z, Y, X), updated in place)."""
+    if ID not in _LATS:
+        raise FlofError(-6, "Load-advect data id %d not initialized!" % ID)
+    dz, dy, dx = dst.shape
+    check(_lib.lib().flof_warp_state_run(
+        _LATS[ID], dst.ptr, dx, dy, dz, phi.h, time, blendAlpha, loadTimeScale,
+        floats(*defoOffset), floats(*defoScale), floats(*defoFactor),
+        floats(*overrideSize), overrideTimeOff, int(zeroVel), bordSkip, defoAniFac),
+        phi.ctx)
+
+
+def loadAdvectTimeSlice_Finish(ID):
+    """optflow4d.cpp:1930-1950."""
+    h = _LATS.pop(ID, None)
+    if h is not None:
+        _lib.lib().flof_warp_state_close(h)
 
 
 def dropSdf4d(phi, seed, nspheres=1, rmin=0.15, rmax=0.15, clampCells=4.):

@@ -1170,6 +1170,86 @@ void dropSdf(R *out, int X, int Y, int Z, int T, int scale, uint64_t seed,
     }
 }
 
+// ---------------------------------------------------------------------------
+// input preprocessing (optflow4d.cpp:1254-1266, 1323-1406)
+
+// repeatFrame4d: slices t in [int(srct - range + 0.5), int(srct)] take the
+// quadrilinear sample at time srct of every (bnd-interior) cell; loops run in
+// the reference's order (t, k, j, i) and update phi in place.
+template <class R>
+void repeatFrame4d(R *phi, const Dims &D, R srct, R range, int bnd)
+{
+    for (int t = int(srct - range + (R)0.5); t <= int(srct); t++)
+        for (int k = bnd; k < D.Z - bnd; ++k)
+            for (int j = bnd; j < D.Y - bnd; ++j)
+                for (int i = bnd; i < D.X - bnd; ++i)
+                    phi[D.at(i, j, k, t)] = interp4<R, R>(
+                        phi, D, (R)i + (R)0.5, (R)j + (R)0.5, (R)k + (R)0.5,
+                        srct + (R)0.5);
+}
+
+// extrap4dLsSimple: mark the inside (phi < 0, or > 0 with `inside`), the
+// first layer around it, then `distance - 1` layers each taking the mean of
+// the previous layer's neighbours plus `direction`; unreached cells get
+// direction*(distance+2).  Neighbour order x-, x+, y-, y+, z-, z+, t-, t+.
+template <class R>
+void extrap4dLsSimple(R *phi, const Dims &D, int distance, bool inside)
+{
+    std::vector<int> tmp(D.n(), 0);
+    const idx_t off[8] = {-1, 1, -(idx_t)D.X, (idx_t)D.X, -D.sz(), D.sz(), -D.st(), D.st()};
+    R direction = 1;
+    if (inside) direction = -1;
+    for (int t = 1; t < D.T - 1; ++t)
+        for (int k = 1; k < D.Z - 1; ++k)
+            for (int j = 1; j < D.Y - 1; ++j)
+                for (int i = 1; i < D.X - 1; ++i) {
+                    const idx_t c = D.at(i, j, k, t);
+                    if (!inside ? phi[c] < 0. : phi[c] > 0.) tmp[c] = 1;
+                }
+    for (int t = 1; t < D.T - 1; ++t)
+        for (int k = 1; k < D.Z - 1; ++k)
+            for (int j = 1; j < D.Y - 1; ++j)
+                for (int i = 1; i < D.X - 1; ++i) {
+                    const idx_t c = D.at(i, j, k, t);
+                    if (tmp[c]) continue;
+                    for (int n = 0; n < 8; ++n)
+                        if (tmp[c + off[n]] == 1) {
+                            tmp[c] = 2;
+                            break;
+                        }
+                }
+    for (int d = 2; d < 1 + distance; ++d) {
+        // cells of layer d+1 read only cells of layer d: order independent
+#pragma omp parallel for schedule(static)
+        for (int t = 1; t < D.T - 1; ++t)
+            for (int k = 1; k < D.Z - 1; ++k)
+                for (int j = 1; j < D.Y - 1; ++j)
+                    for (int i = 1; i < D.X - 1; ++i) {
+                        const idx_t c = D.at(i, j, k, t);
+                        if (tmp[c] != 0) continue;
+                        int nbs = 0;
+                        R avg = 0;
+                        for (int n = 0; n < 8; ++n)
+                            if (tmp[c + off[n]] == d) {
+                                avg += phi[c + off[n]];
+                                nbs++;
+                            }
+                        if (nbs > 0) {
+                            tmp[c] = d + 1;
+                            phi[c] = avg / (R)nbs + direction;
+                        }
+                    }
+    }
+    const R set = direction * (R)(distance + 2);
+    for (int t = 1; t < D.T - 1; ++t)
+        for (int k = 1; k < D.Z - 1; ++k)
+            for (int j = 1; j < D.Y - 1; ++j)
+                for (int i = 1; i < D.X - 1; ++i) {
+                    const idx_t c = D.at(i, j, k, t);
+                    if (tmp[c] == 0) phi[c] = set;
+                }
+}
+
 }  // namespace orc
 
 // ===========================================================================
@@ -1339,6 +1419,18 @@ using namespace orc;
                      (R)loadTimeScale, v4(defoOffset), v4(defoScale),          \
                      v4(defoFactor), v4(overrideSize), (R)overrideTimeOff,     \
                      zeroVel != 0, bordSkip, (R)defoAniFac, optimized != 0);   \
+    }                                                                          \
+    extern "C" void orc_repeat_frame4d##SUF(R *phi, int X, int Y, int Z, int T,  \
+                                            double srct, double range,         \
+                                            int bnd)                           \
+    {                                                                          \
+        repeatFrame4d<R>(phi, Dims{X, Y, Z, T}, (R)srct, (R)range, bnd);       \
+    }                                                                          \
+    extern "C" void orc_extrap4d_ls_simple##SUF(R *phi, int X, int Y, int Z,   \
+                                                int T, int distance,           \
+                                                int inside)                    \
+    {                                                                          \
+        extrap4dLsSimple<R>(phi, Dims{X, Y, Z, T}, distance, inside != 0);     \
     }                                                                          \
     extern "C" void orc_drop_sdf##SUF(R *out, int X, int Y, int Z, int T,      \
                                       int scale, uint64_t seed, int nspheres,  \

@@ -112,6 +112,8 @@ def _declare(L):
                                 _D, _I], None),
             "orc_drop_sdf": ([_P, _I, _I, _I, _I, _I, C.c_uint64, _I, _D, _D,
                               _D], None),
+            "orc_repeat_frame4d": ([_P, _I, _I, _I, _I, _D, _D, _I], None),
+            "orc_extrap4d_ls_simple": ([_P, _I, _I, _I, _I, _I, _I], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name + suf)
@@ -365,6 +367,20 @@ def drop_sdf(X, Y, Z, T, seed, nspheres=1, rmin=0.15, rmax=0.15,
     _fn("orc_drop_sdf", out)(_ptr(out), X, Y, Z, T, scale, seed, nspheres,
                              rmin, rmax, clampCells)
     return out
+
+
+def repeat_frame4d(phi, srct, range_=0.0, bnd=0):
+    """repeatFrame4d (optflow4d.cpp:1254-1266); returns the updated copy."""
+    g = np.array(phi, copy=True, order="C")
+    _fn("orc_repeat_frame4d", g)(_ptr(g), *_dims(g), srct, range_, bnd)
+    return g
+
+
+def extrap4d_ls_simple(phi, distance=4, inside=False):
+    """extrap4dLsSimple (optflow4d.cpp:1361-1406); returns the updated copy."""
+    g = np.array(phi, copy=True, order="C")
+    _fn("orc_extrap4d_ls_simple", g)(_ptr(g), *_dims(g), distance, int(inside))
+    return g
 
 
 P_PLAIN = dict(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5, multiStep=1,

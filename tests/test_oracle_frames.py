@@ -151,3 +151,48 @@ def test_warp_slice_time_blend():
     out1 = O.warp_slice(defo, phi2, dst0, time, blendAlpha=1.0, optimized=False)
     b1 = (slice(1, -1),) * 3
     np.testing.assert_allclose(out1[b1], mc4(phi2, pos)[b1], atol=1e-12)
+
+
+def test_repeat_frame_closed_form():
+    """repeatFrame4d: slices int(srct-range+0.5)..int(srct) take the linear
+    blend of slices floor(srct), floor(srct)+1 at every bnd-interior cell."""
+    T, N = 8, 6
+    phi = RNG.standard_normal((T, N, N, N))
+    out = O.repeat_frame4d(phi, 3.0, 2.0, 0)
+    ref = phi.copy()
+    ref[1:4] = phi[3]
+    np.testing.assert_allclose(out, ref, atol=0)
+    out = O.repeat_frame4d(phi, 2.25, 1.0, 1)
+    ref = phi.copy()
+    blend = 0.75 * phi[2] + 0.25 * phi[3]
+    for t in (1, 2):
+        ref[t, 1:-1, 1:-1, 1:-1] = blend[1:-1, 1:-1, 1:-1]
+    np.testing.assert_allclose(out, ref, atol=1e-14)
+
+
+def test_extrap_planar_closed_form():
+    """extrap4dLsSimple outside a planar level set x < a: the first layer keeps
+    its values, layer d (3 <= d <= distance+1) holds first-layer value +
+    (d - 2), cells beyond get distance + 2; inside is untouched."""
+    T, Z, Y, X = 6, 6, 6, 16
+    x = np.arange(X) + 0.5
+    phi = np.broadcast_to(x - 4.2, (T, Z, Y, X)).copy()  # inside: i <= 3
+    dist = 6
+    out = O.extrap4d_ls_simple(phi, distance=dist, inside=False)
+    core = (slice(1, -1),) * 3
+    first = phi[(slice(1, -1),) * 3 + (4,)]
+    for i in range(1, X - 1):
+        got = out[core + (i,)]
+        if i <= 3:
+            np.testing.assert_array_equal(got, phi[core + (i,)])
+        elif i == 4:
+            np.testing.assert_array_equal(got, first)
+        elif i <= dist + 3:  # passes d = 2..dist fill layers 3..dist+1
+            np.testing.assert_allclose(got, first + (i - 4), atol=1e-12)
+        else:
+            assert (got == dist + 2).all()
+    # inside=True mirrors: the inside region (phi <= 0 here) is extrapolated
+    # with direction -1 from the first layer inside the interface
+    out_in = O.extrap4d_ls_simple(phi, distance=dist, inside=True)
+    first_in = phi[core + (3,)]
+    np.testing.assert_allclose(out_in[core + (2,)], first_in - 1, atol=1e-12)
