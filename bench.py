@@ -6,7 +6,7 @@ A step is one device-resident multiscale FlOF solve at 64^4 (seeded drop
 SDFs, wSmooth 0.1, wEnergy 1e-4, 3 levels, every level run to relative
 signed-max residual 1e-5 -- the iteration cap is raised above the
 reference's 1000 so the residual is the same at every level).  The JSON line
-reports solves/s, the HBM roofline of the dominant kernel (k_cg_update),
+reports solves/s, the HBM roofline of the dominant kernel (k_cg_update_tm),
 timed live with HIP events on the library's stream, and the CPU oracle timed
 on a bounded sample of the same workload on this host.
 """
@@ -29,11 +29,12 @@ PARAMS = dict(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5, minGridSize=16,
               multiStep=1, postVelBlur=0.0, doFinalProject=False,
               resetBndWidth=-1.0, cfl=999.0)
 FIXED_RESIDUAL_CAP = 20000
-# algorithmic bytes per cell of one launch (FP32 Vec4 = 16 B per vector)
-B_UPDATE = 16 * 3 + 16       # read r,q,diag^-1; write r
-B_APPLY = 16 * 2 + 16        # read p,g; write q
-B_DIR = 16 * 4 + 16 * 2      # read x,p,r,diag^-1; write x,p
-B_ITER_MIN = 112             # each of x,r,p read+written once, g read once
+# algorithmic bytes per cell of one launch (FP32 Vec4 = 16 B per vector,
+# i1 = 4 B; g and diag^-1 are recomputed from i1)
+B_UPDATE = 16 * 3 + 4 + 16 * 2   # read p,r,x,i1; write r,x
+B_APPLY = 16 + 4                 # read p,i1 (p.Ap only)
+B_DIR = 16 * 2 + 4 + 16          # read r,p,i1; write p
+B_ITER_MIN = 112                 # x,r,p read+written once, g read once
 
 
 def log(*a):
@@ -117,6 +118,15 @@ def main():
     cell_iters = sum(i * c for i, c in zip(iters, cells))
     ms_step = 1000.0 * float(np.median(times))
     solves_per_s = 1000.0 / ms_step
+    # the same solves without per-launch events (iterations enqueued as
+    # captured graphs): the production launch path, for comparison
+    t_plain = []
+    for _ in range(2):
+        M.sync()
+        t0 = time.perf_counter()
+        solve(FIXED_RESIDUAL_CAP)
+        M.sync()
+        t_plain.append(time.perf_counter() - t0)
 
     # roofline of the dominant kernel: algorithmic bytes of all real launches
     # over the summed event time of every launch of that kernel
@@ -125,11 +135,11 @@ def main():
         byts = bpc * cell_iters * args.steps
         return {"launches": n, "ms_total": ms, "avg_us": 1000.0 * ms / max(1, n),
                 "GBps": byts / (ms * 1e-3) / 1e9 if ms > 0 else None}
-    kst = {"cg_update": kstat(FLOF_KT_CG_UPDATE, B_UPDATE),
-           "cg_apply": kstat(FLOF_KT_CG_APPLY, B_APPLY),
-           "cg_dir": kstat(FLOF_KT_CG_DIR, B_DIR)}
+    kst = {"cg_update_tm": kstat(FLOF_KT_CG_UPDATE, B_UPDATE),
+           "cg_pq_tm": kstat(FLOF_KT_CG_APPLY, B_APPLY),
+           "cg_dir_tm": kstat(FLOF_KT_CG_DIR, B_DIR)}
     dom = max(kst, key=lambda k: kst[k]["ms_total"])
-    bpc = {"cg_update": B_UPDATE, "cg_apply": B_APPLY, "cg_dir": B_DIR}[dom]
+    bpc = {"cg_update_tm": B_UPDATE, "cg_pq_tm": B_APPLY, "cg_dir_tm": B_DIR}[dom]
     achieved = kst[dom]["GBps"] * 1e9
     iter_ms = sum(kst[k]["ms_total"] for k in kst) / args.steps
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
@@ -152,6 +162,7 @@ def main():
                    "residual_per_level": [l["residual"][0] for l in levels]},
         "roofline": roofline,
         "kernels": kst,
+        "ms_per_step_without_kernel_events": 1000.0 * min(t_plain),
     }
 
     # the same solve with the reference's iteration cap (1000)

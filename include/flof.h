@@ -248,6 +248,15 @@ int flof_advect_time_slice_opt(flof_grid4d *defo, float *dst_dev, int dx,
                                const float overrideSize[4],
                                float overrideTimeOff, int zeroVel,
                                int bordSkip, float defoAniFac);
+/* loadAdvectTimeSlice (optflow4d.cpp:1671-1760), the non-cached variant:
+ * the same lookup over the whole bnd=1 interior of dst (no border skip). */
+int flof_advect_time_slice(flof_grid4d *defo, float *dst_dev, int dx, int dy,
+                           int dz, flof_grid4d *phi, float time,
+                           float blendAlpha, float loadTimeScale,
+                           const float defoOffset[4], const float defoScale[4],
+                           const float defoFactor[4],
+                           const float overrideSize[4], float overrideTimeOff,
+                           int zeroVel, float defoAniFac);
 /* Batched form: warps nframes output frames (times time0 + f*dtime) in one
  * launch; frame f is written at dst_dev + f*dx*dy*dz. */
 int flof_advect_time_slices_opt(flof_grid4d *defo, float *dst_dev, int dx,

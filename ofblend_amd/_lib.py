@@ -84,6 +84,8 @@ SIGNATURES = {
                                 _FP, _F, _I, _F, _I, _F, _F], _I),
     "flof_advect_time_slice_opt": ([_P, _P, _I, _I, _I, _P, _F, _F, _F, _FP,
                                     _FP, _FP, _FP, _F, _I, _I, _F], _I),
+    "flof_advect_time_slice": ([_P, _P, _I, _I, _I, _P, _F, _F, _F, _FP, _FP, _FP,
+                                _FP, _F, _I, _F], _I),
     "flof_advect_time_slices_opt": ([_P, _P, _I, _I, _I, _I, _F, _F, _P, _F,
                                      _F, _FP, _FP, _FP, _I, _F], _I),
     "flof_drop_sdf4d": ([_P, _U64, _I, _D, _D, _D], _I),

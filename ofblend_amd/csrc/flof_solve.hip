@@ -15,13 +15,11 @@
 // applied from g on the fly with the identical FP32 operation order, so
 // q = A p is bit-identical to the reference's applyMat.
 //
-// PCG per iteration (three streaming kernels, two 1-block FP64 finalizers):
-//   k_cg_apply(_tm): q = A p, pq = p.q        -> k_cg_alpha: alpha = sigma/pq
-//   k_cg_update    : r -= alpha q, max(r), z = r/diag, z.r
-//                           -> k_cg_beta: converged? beta = z.r/sigma
-//   k_cg_dir       : x += alpha p, p = z + beta p
-// (x is updated in k_cg_dir, which reads p anyway: 208 B per cell and
-// iteration instead of 224.)
+// PCG per iteration: three t-marching kernels (k_cg_pq_tm, k_cg_update_tm,
+// k_cg_dir_tm, 156 B per cell) with two 1-block FP64 finalizers between them
+// (k_cg_alpha: alpha = sigma / p.q; k_cg_beta: stopping test, beta).  The
+// streaming variant with a q array (k_cg_apply, k_cg_update, k_cg_dir,
+// 208 B per cell) is kept for comparison (FLOF_NO_TMARCH=1).
 // The host enqueues iterations in chunks and reads the device state once per
 // chunk; kernels of iterations past convergence return immediately.
 
@@ -394,9 +392,9 @@ k_cg_apply(const float4 *__restrict__ p, const float4 *__restrict__ g,
     if (threadIdx.x == 0) part_dd[blockIdx.x] = acc;
 }
 
-// q = A p marching along t.  Work item = (3D cell c3, t segment); the thread
-// keeps p(t-1), p(t), p(t+1) in registers, so every p value is fetched once
-// from HBM; the x/y/z neighbours at time t are fetched through L2, which holds
+// t-marching work items: (3D cell c3, t segment); the thread keeps the
+// t-1, t, t+1 values of its stencil operands in registers, so every value is
+// fetched once from HBM; the x/y/z neighbours at time t are fetched through L2, which holds
 // the few t-planes the co-resident workgroups are working on.  Workgroups of
 // one XCD (blockIdx % 8) take a contiguous range of work items, so the
 // neighbouring rows and planes of a workgroup live in the same L2.
@@ -407,83 +405,6 @@ struct TMarch {
     int nseg;        // segments
     int total;       // nblk3 * nseg real workgroups
 };
-
-__global__ void __launch_bounds__(256)
-k_cg_apply_tm(const float4 *__restrict__ p, const float4 *__restrict__ g,
-              float4 *__restrict__ q, LevelGeom G, TMarch M, const CgState *st,
-              DD *part_dd)
-{
-    if (st->done) return;
-    const Dims4 &D = G.D;
-    // logical workgroup id: contiguous per XCD
-    const int per = gridDim.x / FLOF_NXCD;
-    const int L = (blockIdx.x % FLOF_NXCD) * per + blockIdx.x / FLOF_NXCD;
-    DD acc{0., 0.};
-    if (L < M.total) {
-        const int seg = L / M.nblk3;
-        const int b3 = L - seg * M.nblk3;
-        const int64_t c3 = (int64_t)b3 * blockDim.x + threadIdx.x;
-        const int t0 = seg * M.seglen;
-        const int t1 = min(D.T, t0 + M.seglen);
-        if (c3 < M.n3 && t0 < t1) {
-            const int i = (int)(c3 % D.X);
-            const int j = (int)((c3 / D.X) % D.Y);
-            const int k = (int)(c3 / D.sz);
-            const bool sp_int = i >= 1 && j >= 1 && k >= 1 && i < D.X - 1 && j < D.Y - 1 &&
-                                k < D.Z - 1;
-            const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            float4 pm = t0 > 0 ? p[c3 + D.st * (t0 - 1)] : zero4;
-            float4 pc = p[c3 + D.st * t0];
-            for (int t = t0; t < t1; ++t) {
-                const int64_t c = c3 + D.st * t;
-                const float4 pp = t + 1 < D.T ? p[c + D.st] : zero4;
-                float4 qc;
-                if (sp_int && t >= 1 && t < D.T - 1) {
-                    const float4 gc = g[c];
-                    float4 v = zero4;
-                    if (G.use_offd) {
-                        const float4 nb[8] = {pm,
-                                              p[c - D.sz],
-                                              p[c - D.X],
-                                              p[c - 1],
-                                              p[c + 1],
-                                              p[c + D.X],
-                                              p[c + D.sz],
-                                              pp};
-#pragma unroll
-                        for (int m = 0; m < 8; ++m) {
-                            v.x = v.x + G.offd * nb[m].x;
-                            v.y = v.y + G.offd * nb[m].y;
-                            v.z = v.z + G.offd * nb[m].z;
-                            v.w = v.w + G.offd * nb[m].w;
-                        }
-                    }
-                    const float gg[4] = {gc.x, gc.y, gc.z, gc.w};
-                    const float pv[4] = {pc.x, pc.y, pc.z, pc.w};
-                    float o[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                    for (int d = 0; d < 4; ++d) {
-#pragma unroll
-                        for (int m = 0; m < 4; ++m) {
-                            float b = gg[d] * gg[m];
-                            if (d == m) b = b + G.diag;
-                            o[d] = o[d] + b * pv[m];
-                        }
-                    }
-                    qc = make_float4(o[0], o[1], o[2], o[3]);
-                } else {
-                    qc = apply_I(pc);
-                }
-                q[c] = qc;
-                dd_add4(acc, pc, qc);
-                pm = pc;
-                pc = pp;
-            }
-        }
-    }
-    acc = block_dd(acc);
-    if (threadIdx.x == 0) part_dd[blockIdx.x] = acc;
-}
 
 // alpha = sigma / p.q
 __global__ void k_cg_alpha(CgState *st, const DD *pd, int nb)
@@ -594,6 +515,236 @@ k_cg_dir(float4 *__restrict__ x, float4 *__restrict__ p, const float4 *__restric
             np.w = (float)((double)z.w + beta * (double)pc.w);
             p[c] = np;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused t-marching PCG (the default).  No q array and no stored g / diag^-1:
+// g is recomputed from i1 (4 B per cell instead of 16 B) exactly as the set-up
+// forms it, the Jacobi preconditioner from g (1/d in FP32 is correctly
+// rounded, as is (float)(1./(double)d): with 53 >= 2*24+2 bits the double
+// rounding is innocuous), and q = A p is formed twice, bit-identically: once
+// for p.q and once in the update.  Per cell and iteration:
+//   k_cg_pq_tm     : read p, i1                        20 B  -> alpha
+//   k_cg_update_tm : read p, i1, r, x; write r, x      84 B  -> beta
+//   k_cg_dir_tm    : read r, i1, p; write p            52 B
+// 156 B instead of 208 B of the three streaming kernels.
+
+__device__ __forceinline__ float4 prec_f32(const float4 &g, float diag)
+{
+    return make_float4(1.f / (g.x * g.x + diag), 1.f / (g.y * g.y + diag),
+                       1.f / (g.z * g.z + diag), 1.f / (g.w * g.w + diag));
+}
+
+// one thread's t-segment of one 3D cell
+struct TmCell {
+    int64_t c3;
+    int t0, t1;
+    bool sp_int;  // x, y, z interior
+    bool live;
+};
+
+__device__ __forceinline__ TmCell tm_cell(const Dims4 &D, const TMarch &M)
+{
+    TmCell w;
+    const int per = gridDim.x / FLOF_NXCD;
+    const int L = (blockIdx.x % FLOF_NXCD) * per + blockIdx.x / FLOF_NXCD;
+    w.live = false;
+    w.c3 = 0;
+    w.t0 = w.t1 = 0;
+    w.sp_int = false;
+    if (L >= M.total) return w;
+    const int seg = L / M.nblk3;
+    const int b3 = L - seg * M.nblk3;
+    w.c3 = (int64_t)b3 * blockDim.x + threadIdx.x;
+    w.t0 = seg * M.seglen;
+    w.t1 = min(D.T, w.t0 + M.seglen);
+    if (w.c3 >= M.n3 || w.t0 >= w.t1) return w;
+    const int i = (int)(w.c3 % D.X);
+    const int j = (int)((w.c3 / D.X) % D.Y);
+    const int k = (int)(w.c3 / D.sz);
+    w.sp_int = i >= 1 && j >= 1 && k >= 1 && i < D.X - 1 && j < D.Y - 1 && k < D.Z - 1;
+    w.live = true;
+    return w;
+}
+
+// g at interior cell c; ipm / ipp are i1 at t-1 / t+1 (optflow4d.cpp:425-433)
+__device__ __forceinline__ float4 grad_i1(const float *__restrict__ i1, int64_t c,
+                                          const Dims4 &D, float ipm, float ipp, float dxf)
+{
+    float4 gc;
+    gc.x = (i1[c + 1] - i1[c - 1]) * dxf;
+    gc.y = (i1[c + D.X] - i1[c - D.X]) * dxf;
+    gc.z = (i1[c + D.sz] - i1[c - D.sz]) * dxf;
+    gc.w = (ipp - ipm) * dxf;
+    return gc;
+}
+
+// A p at an interior cell with p(t-1) = pm, p(t) = pc, p(t+1) = pp in registers
+__device__ __forceinline__ float4 apply_A_tm(const float4 *__restrict__ p, int64_t c,
+                                             const float4 &pm, const float4 &pc,
+                                             const float4 &pp, const float4 &gc,
+                                             const LevelGeom &G)
+{
+    const Dims4 &D = G.D;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (G.use_offd) {
+        const float4 nb[8] = {pm,        p[c - D.sz], p[c - D.X], p[c - 1],
+                              p[c + 1],  p[c + D.X],  p[c + D.sz], pp};
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            v.x = v.x + G.offd * nb[m].x;
+            v.y = v.y + G.offd * nb[m].y;
+            v.z = v.z + G.offd * nb[m].z;
+            v.w = v.w + G.offd * nb[m].w;
+        }
+    }
+    const float gg[4] = {gc.x, gc.y, gc.z, gc.w};
+    const float pv[4] = {pc.x, pc.y, pc.z, pc.w};
+    float o[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            float b = gg[d] * gg[m];
+            if (d == m) b = b + G.diag;
+            o[d] = o[d] + b * pv[m];
+        }
+    }
+    return make_float4(o[0], o[1], o[2], o[3]);
+}
+
+// p.(A p)
+__global__ void __launch_bounds__(256)
+k_cg_pq_tm(const float4 *__restrict__ p, const float *__restrict__ i1, LevelGeom G,
+           TMarch M, float dxf, const CgState *st, DD *part_dd)
+{
+    if (st->done) return;
+    const Dims4 &D = G.D;
+    const TmCell w = tm_cell(D, M);
+    DD acc{0., 0.};
+    if (w.live) {
+        const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int64_t c3 = w.c3;
+        float4 pm = w.t0 > 0 ? p[c3 + D.st * (w.t0 - 1)] : zero4;
+        float4 pc = p[c3 + D.st * w.t0];
+        float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
+        float ic = i1[c3 + D.st * w.t0];
+        for (int t = w.t0; t < w.t1; ++t) {
+            const int64_t c = c3 + D.st * t;
+            const bool tn = t + 1 < D.T;
+            const float4 pp = tn ? p[c + D.st] : zero4;
+            const float ip = tn ? i1[c + D.st] : 0.f;
+            float4 qc;
+            if (w.sp_int && t >= 1 && t < D.T - 1)
+                qc = apply_A_tm(p, c, pm, pc, pp, grad_i1(i1, c, D, im, ip, dxf), G);
+            else
+                qc = apply_I(pc);
+            dd_add4(acc, pc, qc);
+            pm = pc;
+            pc = pp;
+            im = ic;
+            ic = ip;
+        }
+    }
+    acc = block_dd(acc);
+    if (threadIdx.x == 0) part_dd[blockIdx.x] = acc;
+}
+
+// x += alpha p, r -= alpha (A p) (optflow4d.cpp:309-310), z = r diag^-1,
+// z.r and the signed max of r (:311-317)
+__global__ void __launch_bounds__(256)
+k_cg_update_tm(float4 *__restrict__ r, float4 *__restrict__ x, const float4 *__restrict__ p,
+               const float *__restrict__ i1, LevelGeom G, TMarch M, float dxf,
+               const CgState *st, DD *part_dd, float *part_max)
+{
+    if (st->done) return;
+    const Dims4 &D = G.D;
+    const TmCell w = tm_cell(D, M);
+    const double alpha = st->alpha, nalpha = -st->alpha;
+    DD sig{0., 0.};
+    float mx = -INFINITY;
+    if (w.live) {
+        const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int64_t c3 = w.c3;
+        float4 pm = w.t0 > 0 ? p[c3 + D.st * (w.t0 - 1)] : zero4;
+        float4 pc = p[c3 + D.st * w.t0];
+        float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
+        float ic = i1[c3 + D.st * w.t0];
+        for (int t = w.t0; t < w.t1; ++t) {
+            const int64_t c = c3 + D.st * t;
+            const bool tn = t + 1 < D.T;
+            const float4 pp = tn ? p[c + D.st] : zero4;
+            const float ip = tn ? i1[c + D.st] : 0.f;
+            float4 qc, pr;
+            if (w.sp_int && t >= 1 && t < D.T - 1) {
+                const float4 gc = grad_i1(i1, c, D, im, ip, dxf);
+                qc = apply_A_tm(p, c, pm, pc, pp, gc, G);
+                pr = prec_f32(gc, G.diag);
+            } else {
+                qc = apply_I(pc);
+                pr = make_float4(1.f, 1.f, 1.f, 1.f);
+            }
+            float4 xc = x[c];
+            xc.x = (float)((double)xc.x + alpha * (double)pc.x);
+            xc.y = (float)((double)xc.y + alpha * (double)pc.y);
+            xc.z = (float)((double)xc.z + alpha * (double)pc.z);
+            xc.w = (float)((double)xc.w + alpha * (double)pc.w);
+            x[c] = xc;
+            float4 rc = r[c];
+            rc.x = (float)((double)rc.x + nalpha * (double)qc.x);
+            rc.y = (float)((double)rc.y + nalpha * (double)qc.y);
+            rc.z = (float)((double)rc.z + nalpha * (double)qc.z);
+            rc.w = (float)((double)rc.w + nalpha * (double)qc.w);
+            r[c] = rc;
+            const float4 z = f4mul(rc, pr);
+            mx = fmaxf(mx, fmaxf(fmaxf(rc.x, rc.y), fmaxf(rc.z, rc.w)));
+            dd_add4(sig, z, rc);
+            pm = pc;
+            pc = pp;
+            im = ic;
+            ic = ip;
+        }
+    }
+    sig = block_dd(sig);
+    mx = block_max(mx);
+    if (threadIdx.x == 0) {
+        part_dd[blockIdx.x] = sig;
+        part_max[blockIdx.x] = mx;
+    }
+}
+
+// p = z + beta p with z = r diag^-1 (optflow4d.cpp:318-323); x was updated
+// in k_cg_update_tm, so only dir_mode 1 has work
+__global__ void __launch_bounds__(256)
+k_cg_dir_tm(float4 *__restrict__ p, const float4 *__restrict__ r,
+            const float *__restrict__ i1, LevelGeom G, TMarch M, float dxf,
+            const CgState *st)
+{
+    if (st->dir_mode != 1) return;
+    const Dims4 &D = G.D;
+    const TmCell w = tm_cell(D, M);
+    if (!w.live) return;
+    const double beta = st->beta;
+    const int64_t c3 = w.c3;
+    float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
+    float ic = i1[c3 + D.st * w.t0];
+    for (int t = w.t0; t < w.t1; ++t) {
+        const int64_t c = c3 + D.st * t;
+        const float ip = t + 1 < D.T ? i1[c + D.st] : 0.f;
+        float4 pr = make_float4(1.f, 1.f, 1.f, 1.f);
+        if (w.sp_int && t >= 1 && t < D.T - 1)
+            pr = prec_f32(grad_i1(i1, c, D, im, ip, dxf), G.diag);
+        const float4 z = f4mul(r[c], pr);
+        const float4 pc = p[c];
+        float4 np;
+        np.x = (float)((double)z.x + beta * (double)pc.x);
+        np.y = (float)((double)z.y + beta * (double)pc.y);
+        np.z = (float)((double)z.z + beta * (double)pc.z);
+        np.w = (float)((double)z.w + beta * (double)pc.w);
+        p[c] = np;
+        im = ic;
+        ic = ip;
     }
 }
 
@@ -979,18 +1130,26 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
         const int kd = with_timers ? FLOF_KT_CG_DIR : -1;
         for (int k = 0; k < count; ++k) {
             if (ctx->use_tmarch) {
-                klaunch(ctx, ka, k_cg_apply_tm, dim3(W.nb_tm), dim3(256),
-                        (const float4 *)W.p.as<float4>(), (const float4 *)W.g.as<float4>(),
-                        W.q.as<float4>(), G, W.tm, (const CgState *)st, W.pdd());
+                klaunch(ctx, ka, k_cg_pq_tm, dim3(W.nb_tm), dim3(256),
+                        (const float4 *)W.p.as<float4>(), i1, G, W.tm, dxf,
+                        (const CgState *)st, W.pdd());
                 hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st,
                                    W.pdd(), W.nb_tm);
-            } else {
-                klaunch(ctx, ka, k_cg_apply, dim3(nb), dim3(256),
-                        (const float4 *)W.p.as<float4>(), (const float4 *)W.g.as<float4>(),
-                        W.q.as<float4>(), G, (const CgState *)st, W.pdd());
-                hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st,
-                                   W.pdd(), nb);
+                klaunch(ctx, ku, k_cg_update_tm, dim3(W.nb_tm), dim3(256), W.r.as<float4>(),
+                        W.x.as<float4>(), (const float4 *)W.p.as<float4>(), i1, G, W.tm,
+                        dxf, (const CgState *)st, W.pdd(), W.pmax());
+                hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, ctx->stream, st,
+                                   W.pdd(), W.pmax(), W.nb_tm);
+                klaunch(ctx, kd, k_cg_dir_tm, dim3(W.nb_tm), dim3(256), W.p.as<float4>(),
+                        (const float4 *)W.r.as<float4>(), i1, G, W.tm, dxf,
+                        (const CgState *)st);
+                continue;
             }
+            klaunch(ctx, ka, k_cg_apply, dim3(nb), dim3(256),
+                    (const float4 *)W.p.as<float4>(), (const float4 *)W.g.as<float4>(),
+                    W.q.as<float4>(), G, (const CgState *)st, W.pdd());
+            hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
+                               nb);
             klaunch(ctx, ku, k_cg_update, dim3(nb), dim3(256), W.r.as<float4>(),
                     (const float4 *)W.q.as<float4>(), (const float4 *)W.prec.as<float4>(), n,
                     (const CgState *)st, W.pdd(), W.pmax());

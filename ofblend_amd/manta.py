@@ -431,16 +431,23 @@ def loadAdvectTimeSliceFromDefo(defo, dst, phi, time, blendAlpha,
                                 loadTimeScale=1., defoOffset=Vec4(0.),
                                 defoScale=Vec4(1.), defoFactor=Vec4(1.),
                                 overrideSize=Vec4(-1.), overrideTimeOff=0.,
-                                zeroVel=False, bordSkip=1, defoAniFac=1.):
-    """loadAdvectTimeSlice_OptRun (optflow4d.cpp:1951-2105) for one
-    deformation volume held in the Grid4Vec4 `defo`; dst is a DeviceArray
-    (Z, Y, X) updated in place."""
+                                zeroVel=False, bordSkip=1, defoAniFac=1., cached=True):
+    """loadAdvectTimeSlice_OptRun (optflow4d.cpp:1951-2105; cached=False:
+    loadAdvectTimeSlice, :1671-1760) for one deformation volume held in the
+    Grid4Vec4 `defo`; dst is a DeviceArray (Z, Y, X) updated in place."""
     dz, dy, dx = dst.shape
-    check(_lib.lib().flof_advect_time_slice_opt(
-        defo.h, dst.ptr, dx, dy, dz, phi.h, time, blendAlpha, loadTimeScale,
-        floats(*defoOffset), floats(*defoScale), floats(*defoFactor),
-        floats(*overrideSize), overrideTimeOff, int(zeroVel), bordSkip,
-        defoAniFac), phi.ctx)
+    L = _lib.lib()
+    if cached:
+        rc = L.flof_advect_time_slice_opt(
+            defo.h, dst.ptr, dx, dy, dz, phi.h, time, blendAlpha, loadTimeScale,
+            floats(*defoOffset), floats(*defoScale), floats(*defoFactor),
+            floats(*overrideSize), overrideTimeOff, int(zeroVel), bordSkip, defoAniFac)
+    else:
+        rc = L.flof_advect_time_slice(
+            defo.h, dst.ptr, dx, dy, dz, phi.h, time, blendAlpha, loadTimeScale,
+            floats(*defoOffset), floats(*defoScale), floats(*defoFactor),
+            floats(*overrideSize), overrideTimeOff, int(zeroVel), defoAniFac)
+    check(rc, phi.ctx)
 
 
 def repeatFrame4d(phi, srct, range=0., bnd=0):
@@ -536,6 +543,21 @@ def loadAdvectTimeSlice_OptRun(ID, fname, dst, phi, time, blendAlpha, loadTimeSc
         floats(*defoOffset), floats(*defoScale), floats(*defoFactor),
         floats(*overrideSize), overrideTimeOff, int(zeroVel), bordSkip, defoAniFac),
         phi.ctx)
+
+
+def loadAdvectTimeSlice(dummyID, fname, dst, phi, time, blendAlpha, loadTimeScale,
+                        defoOffset, defoScale, defoFactor, overrideSize=Vec4(-1.),
+                        overrideTimeOff=0., debugVel=None, debugVelT=None, zeroVel=False,
+                        thirdAlpha=0., bordSkip=1, fourthAlpha=0., defoAniFac=1.):
+    """optflow4d.cpp:1671-1760: the deformation is read from the M4T2 file
+    `fname` on every call; dst is a DeviceArray (Z, Y, X) updated in place."""
+    X, Y, Z, T = getUniFileSize(fname)
+    defo = Grid4Vec4(phi.parent, (X, Y, Z, T))
+    defo.load(fname)
+    loadAdvectTimeSliceFromDefo(defo, dst, phi, time, blendAlpha, loadTimeScale,
+                                defoOffset, defoScale, defoFactor, overrideSize,
+                                overrideTimeOff, zeroVel, bordSkip, defoAniFac,
+                                cached=False)
 
 
 def loadAdvectTimeSlice_Finish(ID):

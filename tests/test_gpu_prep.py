@@ -155,3 +155,24 @@ def test_advect_time_slice_from_file(tmp_path):
     with pytest.raises(M.FlofError):
         M.loadAdvectTimeSlice_OptRun(7, "", M.DeviceArray.from_numpy(dst0), phi, 0., 1.,
                                      1., M.Vec4(0.), M.Vec4(1.), M.Vec4(1.))
+
+
+def test_advect_time_slice_non_cached(tmp_path):
+    """loadAdvectTimeSlice (file read per call, bnd=1 interior, no border
+    skip) equals the oracle's non-cached warp."""
+    say("loadAdvectTimeSlice (non-cached)")
+    N, T, F = 8, 6, 2
+    frames = O.drop_sdf(N, N, N, T, seed=4, scale=F).astype(np.float32)
+    s = solver(N * F, N * F, N * F, T)
+    phi = real(s, O.place_frames((T, N * F, N * F, N * F), frames))
+    ref_phi = phi.to_numpy()
+    dv = (RNG.standard_normal((T, N, N, N, 4)) * 1.2).astype(np.float32)
+    M.writeUni4d(tmp_path / "defo.uni", dv)
+    dst0 = RNG.standard_normal((N * F,) * 3).astype(np.float32)
+    for time in (0.5, 3.0):
+        dst = M.DeviceArray.from_numpy(dst0)
+        M.loadAdvectTimeSlice(0, str(tmp_path / "defo.uni"), dst, phi, time, 0.8, 1.0,
+                              M.Vec4(0.), M.Vec4(1.), M.Vec4(2, 2, 2, 1))
+        ref = O.warp_slice(dv, ref_phi, dst0, time, blendAlpha=0.8,
+                           defoFactor=(2, 2, 2, 1), optimized=False)
+        assert np.array_equal(dst.to_numpy(), ref)
