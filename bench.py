@@ -49,6 +49,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-frames", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="enqueue CG iterations as plain launches instead of "
+                         "captured graphs (for profilers that trace graphs poorly)")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="short workload for rocprofv3 --pmc runs: a 256 MiB "
                          "grid copy (calibration) and 3 level-0 PCG iterations")
@@ -59,6 +62,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.no_graphs:
+        os.environ["FLOF_NO_GRAPH"] = "1"  # read when the context is created
     from ofblend_amd import _lib
     from ofblend_amd import manta as M
     from ofblend_amd.abi import FLOF_KT_CG_APPLY, FLOF_KT_CG_DIR, FLOF_KT_CG_UPDATE, FLOF_KT_WARP

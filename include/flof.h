@@ -129,6 +129,14 @@ enum flof_kernel_kind {
 int flof_ctx_enable_timing(flof_ctx *ctx, int kind, int enable);
 int flof_ctx_timing(flof_ctx *ctx, int kind, int64_t *launches, double *ms_total);
 
+/* Autotuner of the PCG kernels' launch shape (no reference counterpart):
+ * while enabled, the first solve of each grid shape times the candidate
+ * workgroups-per-CU counts of the t-marching kernels and keeps the fastest
+ * for that shape (results do not depend on it).  flof_ctx_tuned_blocks
+ * returns the choice for a shape, or 0 if it has not been tuned. */
+int flof_ctx_set_autotune(flof_ctx *ctx, int enable);
+int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T);
+
 /* ---- Grid4d storage (source/grid4d.h:117-265, source/grid4d.cpp:60-138) - */
 /* Creates a zero-initialised X*Y*Z*T grid (x fastest, then y, z, t). */
 int flof_grid4d_create(flof_ctx *ctx, int X, int Y, int Z, int T, int elem,
@@ -216,6 +224,19 @@ int flof_optical_flow_multiscale4d(flof_grid4d *vel, flof_grid4d *i0,
                                    const flof_of_params *p,
                                    flof_grid4d *warm, flof_solve_info *info,
                                    float *err);
+/* Batched opticalFlowMultiscale4d (no reference counterpart; the reference
+ * runs its solves one after another): n independent solves, sizes may
+ * differ, all with parameters *p (warmStart must be 0), run on
+ * `concurrency` worker streams of ctx's device (<= 0: 4, at most 16).
+ * Each result is identical to a flof_optical_flow_multiscale4d call.
+ * infos and errs may be null, else arrays of n. */
+int flof_optical_flow_multiscale4d_batch(flof_ctx *ctx, int n,
+                                         flof_grid4d *const *vel,
+                                         flof_grid4d *const *i0,
+                                         flof_grid4d *const *i1,
+                                         const flof_of_params *p,
+                                         int concurrency,
+                                         flof_solve_info *infos, float *errs);
 /* corrVelsOf4d (optflow4d.cpp:651-802,2121-2130). */
 int flof_corr_vels_of4d(flof_grid4d *dst, flof_grid4d *vel,
                         flof_grid4d *phiOrg, flof_grid4d *phiCurr,

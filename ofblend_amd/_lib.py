@@ -43,6 +43,8 @@ SIGNATURES = {
     "flof_ctx_trim": ([_P], _I),
     "flof_ctx_enable_timing": ([_P, _I, _I], _I),
     "flof_ctx_timing": ([_P, _I, _I64P, _DP], _I),
+    "flof_ctx_set_autotune": ([_P, _I], _I),
+    "flof_ctx_tuned_blocks": ([_P, _I, _I, _I, _I], _I),
     "flof_grid4d_create": ([_P, _I, _I, _I, _I, _I, _I, C.POINTER(_P)], _I),
     "flof_grid4d_destroy": ([_P], _I),
     "flof_grid4d_dims": ([_P, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -79,6 +81,9 @@ SIGNATURES = {
                              C.POINTER(SolveInfo)], _I),
     "flof_optical_flow_multiscale4d": ([_P, _P, _P, _P, C.POINTER(OfParams), _P,
                                         C.POINTER(SolveInfo), _FP], _I),
+    "flof_optical_flow_multiscale4d_batch": ([_P, _I, C.POINTER(_P), C.POINTER(_P),
+                                              C.POINTER(_P), C.POINTER(OfParams), _I,
+                                              C.POINTER(SolveInfo), _FP], _I),
     "flof_corr_vels_of4d": ([_P, _P, _P, _P, _P, _F, _F, _F, _F, _I], _I),
     "flof_load_place_grid4d": ([_P, _I, _I, _I, _P, _FP, _FP, _I, _I, _I, _F,
                                 _FP, _F, _I, _F, _I, _F, _F], _I),

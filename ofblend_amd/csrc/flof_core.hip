@@ -870,6 +870,8 @@ int flof_ctx_create(flof_ctx **out, int device)
 int flof_ctx_destroy(flof_ctx *ctx)
 {
     if (!ctx) return FLOF_ERR_ARG;
+    for (flof_ctx *w : ctx->workers) flof_ctx_destroy(w);
+    ctx->workers.clear();
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &t : ctx->timers)
         for (auto &ev : t.pending) {
@@ -882,6 +884,21 @@ int flof_ctx_destroy(flof_ctx *ctx)
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return FLOF_OK;
+}
+
+int flof_ctx_set_autotune(flof_ctx *ctx, int enable)
+{
+    if (!ctx) return FLOF_ERR_ARG;
+    ctx->autotune = enable != 0;
+    for (flof_ctx *w : ctx->workers) w->autotune = ctx->autotune;
+    return FLOF_OK;
+}
+
+int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T)
+{
+    if (!ctx) return FLOF_ERR_ARG;
+    auto it = ctx->tm_tuned.find(std::array<int, 4>{X, Y, Z, T});
+    return it == ctx->tm_tuned.end() ? 0 : it->second;
 }
 
 const char *flof_last_error(const flof_ctx *ctx)

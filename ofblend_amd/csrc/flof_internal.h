@@ -13,6 +13,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <functional>
+#include <array>
+#include <map>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -83,6 +85,11 @@ struct flof_ctx {
     // device scratch for reductions (partials + state), grown on demand
     void *red = nullptr;
     size_t red_bytes = 0;
+    // worker contexts of batched solves (same device, own streams)
+    std::vector<flof_ctx *> workers;
+    // t-march autotuner: on/off and the chosen workgroups per CU per shape
+    bool autotune = false;
+    std::map<std::array<int, 4>, int> tm_tuned;
 };
 
 struct flof_warp_state {

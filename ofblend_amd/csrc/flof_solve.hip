@@ -24,6 +24,9 @@
 // chunk; kernels of iterations past convergence return immediately.
 
 #include <algorithm>
+#include <array>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -1007,12 +1010,12 @@ struct SolveOut {
 
 // t-march decomposition: enough workgroups to fill the chip, t split into
 // segments when one t-slab has too few cells
-static TMarch make_tmarch(flof_ctx *ctx, const Dims4 &D)
+static TMarch make_tmarch(flof_ctx *ctx, const Dims4 &D, int blocks_per_cu)
 {
     TMarch M;
     M.n3 = D.st;
     M.nblk3 = (int)((M.n3 + 255) / 256);
-    const int target = ctx->num_cu * ctx->tm_blocks_per_cu;
+    const int target = ctx->num_cu * blocks_per_cu;
     int nseg = (target + M.nblk3 - 1) / M.nblk3;
     if (nseg < 1) nseg = 1;
     if (nseg > D.T) nseg = D.T;
@@ -1022,12 +1025,20 @@ static TMarch make_tmarch(flof_ctx *ctx, const Dims4 &D)
     return M;
 }
 
+// candidates of the t-march autotuner (workgroups per CU)
+static const int TM_CANDIDATES[] = {2, 4, 8, 16};
+
 struct LevelWork {
     Tmp g, prec, x, r, p, q, st, part;
     int nb = 0;     // grid of the streaming kernels
-    int nb_tm = 0;  // grid of the t-marching apply
+    int nb_tm = 0;  // grid of the t-marching kernels
     int cap = 0;    // partial slots
     TMarch tm;
+    void set_tm(flof_ctx *ctx, const Dims4 &D, int bpc)
+    {
+        tm = make_tmarch(ctx, D, bpc);
+        nb_tm = ((tm.total + FLOF_NXCD - 1) / FLOF_NXCD) * FLOF_NXCD;
+    }
     LevelWork(flof_ctx *ctx, const Dims4 &D)
     {
         const int64_t n = (int64_t)D.st * D.T;
@@ -1039,9 +1050,11 @@ struct LevelWork {
         p = Tmp(ctx, b);
         q = Tmp(ctx, b);
         nb = grid_blocks_xcd(ctx, n);
-        tm = make_tmarch(ctx, D);
-        nb_tm = ((tm.total + FLOF_NXCD - 1) / FLOF_NXCD) * FLOF_NXCD;
+        // partial slots for the largest t-march grid the autotuner may pick
+        set_tm(ctx, D, TM_CANDIDATES[3]);
         cap = std::max(nb, nb_tm);
+        set_tm(ctx, D, ctx->tm_blocks_per_cu);
+        cap = std::max(cap, nb_tm);
         st = Tmp(ctx, sizeof(CgState));
         part = Tmp(ctx, (size_t)cap * (2 * sizeof(DD) + sizeof(float)) + 256);
         FLOF_HIP(hipMemsetAsync(st.p, 0, sizeof(CgState), ctx->stream));
@@ -1050,6 +1063,69 @@ struct LevelWork {
     DD *pdd2() const { return part.as<DD>() + cap; }
     float *pmax() const { return (float *)(part.as<DD>() + 2 * cap); }
 };
+
+// Autotuner of the t-march decomposition: on the first solve of a grid
+// shape, times one PCG iteration (the three t-marching kernels, twice) for
+// each candidate number of workgroups per CU on the level's own (not yet
+// initialised) workspace, with a scratch solver state that lets every kernel
+// run, and keeps the fastest per shape for the context's lifetime.  The
+// workspace is written by the set-up kernel afterwards, so the result of the
+// solve does not depend on the tuning.
+static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const float *i1,
+                     const LevelGeom &G, float dxf)
+{
+    const std::array<int, 4> key{D.X, D.Y, D.Z, D.T};
+    auto it = ctx->tm_tuned.find(key);
+    if (it != ctx->tm_tuned.end()) return it->second;
+    CgState hs;
+    std::memset(&hs, 0, sizeof(hs));
+    hs.alpha = 0.;
+    hs.beta = 0.5;
+    hs.dir_mode = 1;
+    Tmp stt(ctx, sizeof(CgState));
+    FLOF_HIP(hipMemcpyAsync(stt.p, &hs, sizeof(hs), hipMemcpyHostToDevice, ctx->stream));
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));  // hs lives on the stack
+    hipEvent_t e0, e1;
+    FLOF_HIP(hipEventCreate(&e0));
+    FLOF_HIP(hipEventCreate(&e1));
+    const CgState *st = stt.as<CgState>();
+    auto iterate = [&]() {
+        hipLaunchKernelGGL(k_cg_pq_tm, dim3(W.nb_tm), dim3(256), 0, ctx->stream,
+                           (const float4 *)W.p.as<float4>(), i1, G, W.tm, dxf, st, W.pdd());
+        hipLaunchKernelGGL(k_cg_update_tm, dim3(W.nb_tm), dim3(256), 0, ctx->stream,
+                           W.r.as<float4>(), W.x.as<float4>(), (const float4 *)W.p.as<float4>(),
+                           i1, G, W.tm, dxf, st, W.pdd(), W.pmax());
+        hipLaunchKernelGGL(k_cg_dir_tm, dim3(W.nb_tm), dim3(256), 0, ctx->stream,
+                           W.p.as<float4>(), (const float4 *)W.r.as<float4>(), i1, G, W.tm,
+                           dxf, st);
+    };
+    int best = ctx->tm_blocks_per_cu;
+    float best_ms = 1e30f;
+    hipError_t err = hipSuccess;
+    for (int bpc : TM_CANDIDATES) {
+        W.set_tm(ctx, D, bpc);
+        if (W.nb_tm > W.cap) continue;
+        iterate();  // warm-up
+        err = hipEventRecord(e0, ctx->stream);
+        iterate();
+        iterate();
+        if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        if (err != hipSuccess) break;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = bpc;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    FLOF_HIP(err);
+    check_launch("autotune");
+    ctx->tm_tuned[key] = best;
+    return best;
+}
 
 // Solves one level.  vin: incoming flow for the rhs (may be null = zero).
 // The result x/h is written to (accumulate ? += : =) out.
@@ -1081,6 +1157,11 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
         FLOF_HIP(hipMemcpyAsync(&st->maxIter, &maxIt, sizeof(int), hipMemcpyHostToDevice,
                                 ctx->stream));
         FLOF_HIP(hipStreamSynchronize(ctx->stream));  // maxIt lives on the stack
+    }
+    if (ctx->use_tmarch) {
+        int bpc = ctx->tm_blocks_per_cu;
+        if (ctx->autotune && n >= (1 << 16)) bpc = tuned_bpc(ctx, D, W, i1, G, dxf);
+        W.set_tm(ctx, D, bpc);
     }
     // with fewer than 3 cells in a dimension every row is an identity row and
     // the neighbour offsets are never used
@@ -1410,6 +1491,44 @@ static float multiscale(flof_ctx *ctx, float4 *vel, const float *i0, const float
     return (float)errCurr;
 }
 
+static MsParams ms_params(const flof_of_params *p)
+{
+    FLOF_REQUIRE(p->blurType == 1 || p->postVelBlur <= 0., FLOF_ERR_ARG, "NYI");
+    MsParams P;
+    P.wSmooth = (float)p->wSmooth;
+    P.wEnergy = (float)p->wEnergy;
+    P.postVelBlur = (float)p->postVelBlur;
+    P.cgAccuracy = (float)p->cgAccuracy;
+    P.cfl = (float)p->cfl;
+    P.resetBndWidth = (float)p->resetBndWidth;
+    P.multiStep = p->multiStep;
+    P.projSizeThresh = p->projSizeThresh;
+    P.minGridSize = p->minGridSize;
+    P.doFinalProject = p->doFinalProject;
+    P.cgMaxIter = p->cgMaxIter;
+    FLOF_REQUIRE(P.multiStep < FLOF_MAX_STEPS, FLOF_ERR_RANGE, "Too many of substeps!");
+    return P;
+}
+
+// Worker contexts of ctx (same device, own stream and pool), created on
+// first use with the parent's launch configuration.
+static flof_ctx *worker_ctx(flof_ctx *ctx, int w)
+{
+    while ((int)ctx->workers.size() <= w) {
+        flof_ctx *c = nullptr;
+        const int rc = flof_ctx_create(&c, ctx->device);
+        FLOF_REQUIRE(rc == FLOF_OK && c, rc ? rc : FLOF_ERR_HIP,
+                     "opticalFlowMultiscale4dBatch: cannot create a worker stream");
+        c->use_graphs = ctx->use_graphs;
+        c->blocks_per_cu = ctx->blocks_per_cu;
+        c->use_tmarch = ctx->use_tmarch;
+        c->tm_blocks_per_cu = ctx->tm_blocks_per_cu;
+        c->autotune = ctx->autotune;
+        ctx->workers.push_back(c);
+    }
+    return ctx->workers[w];
+}
+
 }  // namespace flof
 
 using namespace flof;
@@ -1514,25 +1633,12 @@ int flof_optical_flow_multiscale4d(flof_grid4d *vel, flof_grid4d *i0, flof_grid4
         need_dims(vel, rhsT, "opticalFlowMultiscale4d");
     }
     FLOF_REQUIRE(p, FLOF_ERR_ARG, "opticalFlowMultiscale4d: null params");
-    FLOF_REQUIRE(p->blurType == 1 || p->postVelBlur <= 0., FLOF_ERR_ARG, "NYI");
     if (p->warmStart) {
         FLOF_REQUIRE(warm, FLOF_ERR_ARG, "opticalFlowMultiscale4d: warmStart without warm grid");
         need4(warm, FLOF_VEC4, "opticalFlowMultiscale4d(warm)");
         need_dims(vel, warm, "opticalFlowMultiscale4d");
     }
-    MsParams P;
-    P.wSmooth = (float)p->wSmooth;
-    P.wEnergy = (float)p->wEnergy;
-    P.postVelBlur = (float)p->postVelBlur;
-    P.cgAccuracy = (float)p->cgAccuracy;
-    P.cfl = (float)p->cfl;
-    P.resetBndWidth = (float)p->resetBndWidth;
-    P.multiStep = p->multiStep;
-    P.projSizeThresh = p->projSizeThresh;
-    P.minGridSize = p->minGridSize;
-    P.doFinalProject = p->doFinalProject;
-    P.cgMaxIter = p->cgMaxIter;
-    FLOF_REQUIRE(P.multiStep < FLOF_MAX_STEPS, FLOF_ERR_RANGE, "Too many of substeps!");
+    const MsParams P = ms_params(p);
     flof_solve_info local;
     flof_solve_info *inf = info ? info : &local;
     std::memset(inf, 0, sizeof(*inf));
@@ -1542,6 +1648,78 @@ int flof_optical_flow_multiscale4d(flof_grid4d *vel, flof_grid4d *i0, flof_grid4
                                p->warmStart ? (const float4 *)warm->data : nullptr);
     if (err) *err = e;
     FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    API_CATCH(ctx)
+}
+
+int flof_optical_flow_multiscale4d_batch(flof_ctx *ctx, int n, flof_grid4d *const *vel,
+                                         flof_grid4d *const *i0, flof_grid4d *const *i1,
+                                         const flof_of_params *p, int concurrency,
+                                         flof_solve_info *infos, float *errs)
+{
+    API_TRY
+    FLOF_REQUIRE(ctx && n >= 0 && p && (n == 0 || (vel && i0 && i1)), FLOF_ERR_ARG,
+                 "opticalFlowMultiscale4dBatch: bad argument");
+    FLOF_REQUIRE(!p->warmStart, FLOF_ERR_ARG,
+                 "opticalFlowMultiscale4dBatch: warm starts are per call");
+    for (int j = 0; j < n; ++j) {
+        need4(vel[j], FLOF_VEC4, "opticalFlowMultiscale4dBatch(vel)");
+        need4(i0[j], FLOF_REAL, "opticalFlowMultiscale4dBatch(i0)");
+        need4(i1[j], FLOF_REAL, "opticalFlowMultiscale4dBatch(i1)");
+        need_dims(vel[j], i0[j], "opticalFlowMultiscale4dBatch");
+        need_dims(vel[j], i1[j], "opticalFlowMultiscale4dBatch");
+        FLOF_REQUIRE(vel[j]->ctx->device == ctx->device && i0[j]->ctx->device == ctx->device &&
+                         i1[j]->ctx->device == ctx->device,
+                     FLOF_ERR_ARG, "opticalFlowMultiscale4dBatch: grids on another device");
+        for (int m = 0; m < j; ++m)
+            FLOF_REQUIRE(vel[m] != vel[j], FLOF_ERR_ARG,
+                         "opticalFlowMultiscale4dBatch: one vel grid in two solves");
+    }
+    const MsParams P = ms_params(p);
+    if (n == 0) return FLOF_OK;
+    const int W = std::max(1, std::min(std::min(concurrency > 0 ? concurrency : 4, 16), n));
+    std::vector<flof_ctx *> wk;
+    for (int w = 0; w < W; ++w) wk.push_back(worker_ctx(ctx, w));
+    // inputs may still be written by work enqueued on the grids' streams
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    for (int j = 0; j < n; ++j)
+        for (flof_grid4d *g : {vel[j], i0[j], i1[j]})
+            if (g->ctx != ctx) FLOF_HIP(hipStreamSynchronize(g->ctx->stream));
+    std::atomic<int> next{0};
+    std::vector<int> rc(n, FLOF_OK);
+    std::vector<std::string> msg(n);
+    auto run = [&](flof_ctx *w) {
+        if (hipSetDevice(w->device) != hipSuccess) return;
+        for (;;) {
+            const int j = next.fetch_add(1);
+            if (j >= n) break;
+            try {
+                flof_solve_info local;
+                flof_solve_info *inf = infos ? &infos[j] : &local;
+                std::memset(inf, 0, sizeof(*inf));
+                const float e = multiscale(w, (float4 *)vel[j]->data,
+                                           (const float *)i0[j]->data,
+                                           (const float *)i1[j]->data, nullptr,
+                                           dims_of(vel[j]), P, p->level, 1.f, inf, nullptr);
+                FLOF_HIP(hipStreamSynchronize(w->stream));
+                if (errs) errs[j] = e;
+            } catch (const flof::Error &e) {
+                rc[j] = e.code;
+                msg[j] = e.msg;
+                (void)hipStreamSynchronize(w->stream);
+            } catch (const std::exception &e) {
+                rc[j] = FLOF_ERR_ARG;
+                msg[j] = e.what();
+                (void)hipStreamSynchronize(w->stream);
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < W; ++w) th.emplace_back(run, wk[w]);
+    run(wk[0]);
+    for (auto &t : th) t.join();
+    for (int j = 0; j < n; ++j)
+        FLOF_REQUIRE(rc[j] == FLOF_OK, rc[j],
+                     "opticalFlowMultiscale4dBatch: solve " + std::to_string(j) + ": " + msg[j]);
     API_CATCH(ctx)
 }
 

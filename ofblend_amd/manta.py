@@ -285,6 +285,25 @@ def opticalFlowMultiscale4d(vel, i0, i1, rhsT=None, wSmooth=0., wEnergy=0.,
     return err.value
 
 
+def opticalFlowMultiscale4dBatch(vels, i0s, i1s, concurrency=4, **kw):
+    """Extension: independent opticalFlowMultiscale4d solves (sizes may
+    differ) run concurrently on worker streams; returns (errs, infos), each
+    result identical to a single call with the same keyword arguments."""
+    n = len(vels)
+    if not (len(i0s) == n and len(i1s) == n):
+        raise FlofError(-1, "opticalFlowMultiscale4dBatch: list lengths differ")
+    if kw.pop("warm", None) is not None:
+        raise FlofError(-1, "opticalFlowMultiscale4dBatch: no warm start")
+    p = _params(**{k: (int(v) if k == "doFinalProject" else v) for k, v in kw.items()})
+    ctx = vels[0].ctx if n else context()
+    arr = lambda gs: (C.c_void_p * max(1, n))(*[g.h.value for g in gs])
+    infos = (SolveInfo * max(1, n))()
+    errs = (C.c_float * max(1, n))()
+    check(_lib.lib().flof_optical_flow_multiscale4d_batch(
+        ctx, n, arr(vels), arr(i0s), arr(i1s), C.byref(p), concurrency, infos, errs), ctx)
+    return [errs[j] for j in range(n)], [infos[j] for j in range(n)]
+
+
 def opticalFlow4d(vel, i0, i1, rhsT=None, wSmooth=0., wEnergy=0.,
                   postVelBlur=0., cgAccuracy=1e-04, blurType=1,
                   resetBndWidth=-1., cgMaxIter=1000):
