@@ -78,8 +78,14 @@ typedef struct flof_of_params {
     int32_t cgMaxIter;     /* 1000   CG iteration cap (optflow4d.cpp:502)     */
     /* Extensions of this library (no reference counterpart): */
     int32_t warmStart;     /* 0      1: use *warm as CG start at each level   */
-    int32_t pad_;
+    int32_t precision;     /* 0      storage of the CG search direction:
+                                     FLOF_PSTORE_F32 (the reference's
+                                     arithmetic) or FLOF_PSTORE_BF16 (x, r, g
+                                     and all arithmetic stay FP32, scalars
+                                     FP64)                                    */
 } flof_of_params;
+
+enum flof_pstore { FLOF_PSTORE_F32 = 0, FLOF_PSTORE_BF16 = 1 };
 
 #define FLOF_MAX_LEVELS 16
 #define FLOF_MAX_STEPS 10

@@ -16,6 +16,9 @@ FLOF_F16 = 2
 FLOF_MAX_LEVELS = 16
 FLOF_MAX_STEPS = 10
 
+FLOF_PSTORE_F32 = 0
+FLOF_PSTORE_BF16 = 1
+
 FLOF_KT_CG_APPLY = 0
 FLOF_KT_CG_UPDATE = 1
 FLOF_KT_CG_DIR = 2
@@ -43,7 +46,7 @@ class OfParams(C.Structure):
         ("doFinalProject", C.c_int32),
         ("cgMaxIter", C.c_int32),
         ("warmStart", C.c_int32),
-        ("pad_", C.c_int32),
+        ("precision", C.c_int32),
     ]
 
 
@@ -54,7 +57,7 @@ def default_params(**kw):
                  cgAccuracy=1e-4, blurType=1, cfl=999.0, orderTime=1,
                  orderSpace=1, resetBndWidth=-1.0, multiStep=1,
                  projSizeThresh=9999, minGridSize=10, doFinalProject=0,
-                 cgMaxIter=1000, warmStart=0, pad_=0)
+                 cgMaxIter=1000, warmStart=0, precision=0)
     for k, v in kw.items():
         if not hasattr(p, k):
             raise TypeError("unknown parameter %r" % k)

@@ -897,7 +897,7 @@ int flof_ctx_set_autotune(flof_ctx *ctx, int enable)
 int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T)
 {
     if (!ctx) return FLOF_ERR_ARG;
-    auto it = ctx->tm_tuned.find(std::array<int, 4>{X, Y, Z, T});
+    auto it = ctx->tm_tuned.find(std::array<int, 5>{X, Y, Z, T, ctx->p_store});
     return it == ctx->tm_tuned.end() ? 0 : it->second;
 }
 

@@ -89,7 +89,10 @@ struct flof_ctx {
     std::vector<flof_ctx *> workers;
     // t-march autotuner: on/off and the chosen workgroups per CU per shape
     bool autotune = false;
-    std::map<std::array<int, 4>, int> tm_tuned;
+    std::map<std::array<int, 5>, int> tm_tuned;  // X, Y, Z, T, p storage
+    // storage of the CG search direction for the call in progress
+    // (FLOF_PSTORE_*), set from flof_of_params.precision by the entry points
+    int p_store = FLOF_PSTORE_F32;
 };
 
 struct flof_warp_state {

@@ -583,8 +583,51 @@ __device__ __forceinline__ float4 grad_i1(const float *__restrict__ i1, int64_t 
     return gc;
 }
 
+// Storage of the search direction p.  FP32 reproduces the reference; the
+// bfloat16 form (extension) keeps p as 4 x 16 bit per cell, rounded to
+// nearest even on every store, and every kernel works with the stored
+// (rounded) direction, so x, r and the FP64 scalars stay consistent with it.
+// (binary16 is not offered: p shrinks with the residual and leaves its
+// exponent range long before a solve to 1e-5 ends.)
+struct PStoreF32 {
+    typedef float4 T;
+    static __device__ __forceinline__ float4 ld(const T *p, int64_t c) { return p[c]; }
+    static __device__ __forceinline__ void st(T *p, int64_t c, const float4 &v) { p[c] = v; }
+};
+typedef unsigned short bf4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float bf16_to_f32(unsigned short u)
+{
+    return __uint_as_float((unsigned)u << 16);
+}
+__device__ __forceinline__ unsigned short f32_to_bf16(float f)
+{
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+struct PStoreBF16 {
+    typedef bf4v T;
+    static __device__ __forceinline__ float4 ld(const T *p, int64_t c)
+    {
+        const bf4v h = p[c];
+        return make_float4(bf16_to_f32(h.x), bf16_to_f32(h.y), bf16_to_f32(h.z),
+                           bf16_to_f32(h.w));
+    }
+    static __device__ __forceinline__ void st(T *p, int64_t c, const float4 &v)
+    {
+        bf4v h;
+        h.x = f32_to_bf16(v.x);
+        h.y = f32_to_bf16(v.y);
+        h.z = f32_to_bf16(v.z);
+        h.w = f32_to_bf16(v.w);
+        p[c] = h;
+    }
+};
+
 // A p at an interior cell with p(t-1) = pm, p(t) = pc, p(t+1) = pp in registers
-__device__ __forceinline__ float4 apply_A_tm(const float4 *__restrict__ p, int64_t c,
+template <class PS>
+__device__ __forceinline__ float4 apply_A_tm(const typename PS::T *__restrict__ p, int64_t c,
                                              const float4 &pm, const float4 &pc,
                                              const float4 &pp, const float4 &gc,
                                              const LevelGeom &G)
@@ -592,8 +635,14 @@ __device__ __forceinline__ float4 apply_A_tm(const float4 *__restrict__ p, int64
     const Dims4 &D = G.D;
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (G.use_offd) {
-        const float4 nb[8] = {pm,        p[c - D.sz], p[c - D.X], p[c - 1],
-                              p[c + 1],  p[c + D.X],  p[c + D.sz], pp};
+        const float4 nb[8] = {pm,
+                              PS::ld(p, c - D.sz),
+                              PS::ld(p, c - D.X),
+                              PS::ld(p, c - 1),
+                              PS::ld(p, c + 1),
+                              PS::ld(p, c + D.X),
+                              PS::ld(p, c + D.sz),
+                              pp};
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
             v.x = v.x + G.offd * nb[m].x;
@@ -618,8 +667,9 @@ __device__ __forceinline__ float4 apply_A_tm(const float4 *__restrict__ p, int64
 }
 
 // p.(A p)
+template <class PS>
 __global__ void __launch_bounds__(256)
-k_cg_pq_tm(const float4 *__restrict__ p, const float *__restrict__ i1, LevelGeom G,
+k_cg_pq_tm(const typename PS::T *__restrict__ p, const float *__restrict__ i1, LevelGeom G,
            TMarch M, float dxf, const CgState *st, DD *part_dd)
 {
     if (st->done) return;
@@ -629,18 +679,18 @@ k_cg_pq_tm(const float4 *__restrict__ p, const float *__restrict__ i1, LevelGeom
     if (w.live) {
         const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
         const int64_t c3 = w.c3;
-        float4 pm = w.t0 > 0 ? p[c3 + D.st * (w.t0 - 1)] : zero4;
-        float4 pc = p[c3 + D.st * w.t0];
+        float4 pm = w.t0 > 0 ? PS::ld(p, c3 + D.st * (w.t0 - 1)) : zero4;
+        float4 pc = PS::ld(p, c3 + D.st * w.t0);
         float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
         float ic = i1[c3 + D.st * w.t0];
         for (int t = w.t0; t < w.t1; ++t) {
             const int64_t c = c3 + D.st * t;
             const bool tn = t + 1 < D.T;
-            const float4 pp = tn ? p[c + D.st] : zero4;
+            const float4 pp = tn ? PS::ld(p, c + D.st) : zero4;
             const float ip = tn ? i1[c + D.st] : 0.f;
             float4 qc;
             if (w.sp_int && t >= 1 && t < D.T - 1)
-                qc = apply_A_tm(p, c, pm, pc, pp, grad_i1(i1, c, D, im, ip, dxf), G);
+                qc = apply_A_tm<PS>(p, c, pm, pc, pp, grad_i1(i1, c, D, im, ip, dxf), G);
             else
                 qc = apply_I(pc);
             dd_add4(acc, pc, qc);
@@ -656,10 +706,11 @@ k_cg_pq_tm(const float4 *__restrict__ p, const float *__restrict__ i1, LevelGeom
 
 // x += alpha p, r -= alpha (A p) (optflow4d.cpp:309-310), z = r diag^-1,
 // z.r and the signed max of r (:311-317)
+template <class PS>
 __global__ void __launch_bounds__(256)
-k_cg_update_tm(float4 *__restrict__ r, float4 *__restrict__ x, const float4 *__restrict__ p,
-               const float *__restrict__ i1, LevelGeom G, TMarch M, float dxf,
-               const CgState *st, DD *part_dd, float *part_max)
+k_cg_update_tm(float4 *__restrict__ r, float4 *__restrict__ x,
+               const typename PS::T *__restrict__ p, const float *__restrict__ i1, LevelGeom G,
+               TMarch M, float dxf, const CgState *st, DD *part_dd, float *part_max)
 {
     if (st->done) return;
     const Dims4 &D = G.D;
@@ -670,19 +721,19 @@ k_cg_update_tm(float4 *__restrict__ r, float4 *__restrict__ x, const float4 *__r
     if (w.live) {
         const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
         const int64_t c3 = w.c3;
-        float4 pm = w.t0 > 0 ? p[c3 + D.st * (w.t0 - 1)] : zero4;
-        float4 pc = p[c3 + D.st * w.t0];
+        float4 pm = w.t0 > 0 ? PS::ld(p, c3 + D.st * (w.t0 - 1)) : zero4;
+        float4 pc = PS::ld(p, c3 + D.st * w.t0);
         float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
         float ic = i1[c3 + D.st * w.t0];
         for (int t = w.t0; t < w.t1; ++t) {
             const int64_t c = c3 + D.st * t;
             const bool tn = t + 1 < D.T;
-            const float4 pp = tn ? p[c + D.st] : zero4;
+            const float4 pp = tn ? PS::ld(p, c + D.st) : zero4;
             const float ip = tn ? i1[c + D.st] : 0.f;
             float4 qc, pr;
             if (w.sp_int && t >= 1 && t < D.T - 1) {
                 const float4 gc = grad_i1(i1, c, D, im, ip, dxf);
-                qc = apply_A_tm(p, c, pm, pc, pp, gc, G);
+                qc = apply_A_tm<PS>(p, c, pm, pc, pp, gc, G);
                 pr = prec_f32(gc, G.diag);
             } else {
                 qc = apply_I(pc);
@@ -719,8 +770,9 @@ k_cg_update_tm(float4 *__restrict__ r, float4 *__restrict__ x, const float4 *__r
 
 // p = z + beta p with z = r diag^-1 (optflow4d.cpp:318-323); x was updated
 // in k_cg_update_tm, so only dir_mode 1 has work
+template <class PS>
 __global__ void __launch_bounds__(256)
-k_cg_dir_tm(float4 *__restrict__ p, const float4 *__restrict__ r,
+k_cg_dir_tm(typename PS::T *__restrict__ p, const float4 *__restrict__ r,
             const float *__restrict__ i1, LevelGeom G, TMarch M, float dxf,
             const CgState *st)
 {
@@ -739,16 +791,26 @@ k_cg_dir_tm(float4 *__restrict__ p, const float4 *__restrict__ r,
         if (w.sp_int && t >= 1 && t < D.T - 1)
             pr = prec_f32(grad_i1(i1, c, D, im, ip, dxf), G.diag);
         const float4 z = f4mul(r[c], pr);
-        const float4 pc = p[c];
+        const float4 pc = PS::ld(p, c);
         float4 np;
         np.x = (float)((double)z.x + beta * (double)pc.x);
         np.y = (float)((double)z.y + beta * (double)pc.y);
         np.z = (float)((double)z.z + beta * (double)pc.z);
         np.w = (float)((double)z.w + beta * (double)pc.w);
-        p[c] = np;
+        PS::st(p, c, np);
         im = ic;
         ic = ip;
     }
+}
+
+// the FP32 direction of the set-up (or warm start) into reduced storage
+template <class PS>
+__global__ void __launch_bounds__(256)
+k_p_pack(const float4 *__restrict__ src, typename PS::T *__restrict__ dst, int64_t n)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x)
+        PS::st(dst, c, src[c]);
 }
 
 // vel_out (=) or (+=) x / mDx; a failed solve contributes zero
@@ -1064,6 +1126,39 @@ struct LevelWork {
     float *pmax() const { return (float *)(part.as<DD>() + 2 * cap); }
 };
 
+// one fused PCG iteration (three t-marching kernels and the two finalizers);
+// pbuf holds p in the storage of PS; ka/ku/kd: timer kinds or -1
+template <class PS>
+static void enqueue_tm_iter(flof_ctx *ctx, LevelWork &W, void *pbuf, const float *i1,
+                            const LevelGeom &G, float dxf, CgState *st, int ka, int ku, int kd,
+                            bool finalizers)
+{
+    typedef typename PS::T PT;
+    klaunch(ctx, ka, k_cg_pq_tm<PS>, dim3(W.nb_tm), dim3(256), (const PT *)pbuf, i1, G, W.tm,
+            dxf, (const CgState *)st, W.pdd());
+    if (finalizers)
+        hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
+                           W.nb_tm);
+    klaunch(ctx, ku, k_cg_update_tm<PS>, dim3(W.nb_tm), dim3(256), W.r.as<float4>(),
+            W.x.as<float4>(), (const PT *)pbuf, i1, G, W.tm, dxf, (const CgState *)st, W.pdd(),
+            W.pmax());
+    if (finalizers)
+        hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
+                           W.pmax(), W.nb_tm);
+    klaunch(ctx, kd, k_cg_dir_tm<PS>, dim3(W.nb_tm), dim3(256), (PT *)pbuf,
+            (const float4 *)W.r.as<float4>(), i1, G, W.tm, dxf, (const CgState *)st);
+}
+
+static void enqueue_tm_iter_any(flof_ctx *ctx, int pstore, LevelWork &W, void *pbuf,
+                                const float *i1, const LevelGeom &G, float dxf, CgState *st,
+                                int ka, int ku, int kd, bool finalizers)
+{
+    if (pstore == FLOF_PSTORE_BF16)
+        enqueue_tm_iter<PStoreBF16>(ctx, W, pbuf, i1, G, dxf, st, ka, ku, kd, finalizers);
+    else
+        enqueue_tm_iter<PStoreF32>(ctx, W, pbuf, i1, G, dxf, st, ka, ku, kd, finalizers);
+}
+
 // Autotuner of the t-march decomposition: on the first solve of a grid
 // shape, times one PCG iteration (the three t-marching kernels, twice) for
 // each candidate number of workgroups per CU on the level's own (not yet
@@ -1072,9 +1167,9 @@ struct LevelWork {
 // workspace is written by the set-up kernel afterwards, so the result of the
 // solve does not depend on the tuning.
 static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const float *i1,
-                     const LevelGeom &G, float dxf)
+                     const LevelGeom &G, float dxf, int pstore, void *pbuf)
 {
-    const std::array<int, 4> key{D.X, D.Y, D.Z, D.T};
+    const std::array<int, 5> key{D.X, D.Y, D.Z, D.T, pstore};
     auto it = ctx->tm_tuned.find(key);
     if (it != ctx->tm_tuned.end()) return it->second;
     CgState hs;
@@ -1088,16 +1183,9 @@ static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const float *i
     hipEvent_t e0, e1;
     FLOF_HIP(hipEventCreate(&e0));
     FLOF_HIP(hipEventCreate(&e1));
-    const CgState *st = stt.as<CgState>();
+    CgState *st = stt.as<CgState>();
     auto iterate = [&]() {
-        hipLaunchKernelGGL(k_cg_pq_tm, dim3(W.nb_tm), dim3(256), 0, ctx->stream,
-                           (const float4 *)W.p.as<float4>(), i1, G, W.tm, dxf, st, W.pdd());
-        hipLaunchKernelGGL(k_cg_update_tm, dim3(W.nb_tm), dim3(256), 0, ctx->stream,
-                           W.r.as<float4>(), W.x.as<float4>(), (const float4 *)W.p.as<float4>(),
-                           i1, G, W.tm, dxf, st, W.pdd(), W.pmax());
-        hipLaunchKernelGGL(k_cg_dir_tm, dim3(W.nb_tm), dim3(256), 0, ctx->stream,
-                           W.p.as<float4>(), (const float4 *)W.r.as<float4>(), i1, G, W.tm,
-                           dxf, st);
+        enqueue_tm_iter_any(ctx, pstore, W, pbuf, i1, G, dxf, st, -1, -1, -1, false);
     };
     int best = ctx->tm_blocks_per_cu;
     float best_ms = 1e30f;
@@ -1158,9 +1246,13 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
                                 ctx->stream));
         FLOF_HIP(hipStreamSynchronize(ctx->stream));  // maxIt lives on the stack
     }
+    // search direction storage: W.p (FP32) or, reduced, W.q (the fused
+    // kernels need no q array; the warm start uses it before the pack)
+    const int pstore = ctx->use_tmarch ? ctx->p_store : FLOF_PSTORE_F32;
+    void *pbuf = pstore == FLOF_PSTORE_F32 ? W.p.p : W.q.p;
     if (ctx->use_tmarch) {
         int bpc = ctx->tm_blocks_per_cu;
-        if (ctx->autotune && n >= (1 << 16)) bpc = tuned_bpc(ctx, D, W, i1, G, dxf);
+        if (ctx->autotune && n >= (1 << 16)) bpc = tuned_bpc(ctx, D, W, i1, G, dxf, pstore, pbuf);
         W.set_tm(ctx, D, bpc);
     }
     // with fewer than 3 cells in a dimension every row is an identity row and
@@ -1195,6 +1287,12 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
                            W.pdd(), W.pmax(), nb);
         check_launch("k_warm_adopt_fin");
     }
+    if (pstore == FLOF_PSTORE_BF16) {
+        hipLaunchKernelGGL(k_p_pack<PStoreBF16>, dim3(grid_blocks(ctx, n)), dim3(256), 0,
+                           ctx->stream, (const float4 *)W.p.as<float4>(),
+                           (PStoreBF16::T *)pbuf, n);
+        check_launch("k_p_pack");
+    }
     // Iterations are enqueued in chunks of CHUNK (as one captured hipGraph
     // unless per-kernel timing is on); after each chunk the state is copied to
     // pinned host memory behind an event, and the host waits only for the
@@ -1211,19 +1309,7 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
         const int kd = with_timers ? FLOF_KT_CG_DIR : -1;
         for (int k = 0; k < count; ++k) {
             if (ctx->use_tmarch) {
-                klaunch(ctx, ka, k_cg_pq_tm, dim3(W.nb_tm), dim3(256),
-                        (const float4 *)W.p.as<float4>(), i1, G, W.tm, dxf,
-                        (const CgState *)st, W.pdd());
-                hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st,
-                                   W.pdd(), W.nb_tm);
-                klaunch(ctx, ku, k_cg_update_tm, dim3(W.nb_tm), dim3(256), W.r.as<float4>(),
-                        W.x.as<float4>(), (const float4 *)W.p.as<float4>(), i1, G, W.tm,
-                        dxf, (const CgState *)st, W.pdd(), W.pmax());
-                hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, ctx->stream, st,
-                                   W.pdd(), W.pmax(), W.nb_tm);
-                klaunch(ctx, kd, k_cg_dir_tm, dim3(W.nb_tm), dim3(256), W.p.as<float4>(),
-                        (const float4 *)W.r.as<float4>(), i1, G, W.tm, dxf,
-                        (const CgState *)st);
+                enqueue_tm_iter_any(ctx, pstore, W, pbuf, i1, G, dxf, st, ka, ku, kd, true);
                 continue;
             }
             klaunch(ctx, ka, k_cg_apply, dim3(nb), dim3(256),
@@ -1491,6 +1577,21 @@ static float multiscale(flof_ctx *ctx, float4 *vel, const float *i0, const float
     return (float)errCurr;
 }
 
+// the direction storage of one call (flof_of_params.precision)
+struct PStoreScope {
+    flof_ctx *c;
+    int old;
+    PStoreScope(flof_ctx *c_, int v) : c(c_), old(c_->p_store)
+    {
+        FLOF_REQUIRE(v == FLOF_PSTORE_F32 || v == FLOF_PSTORE_BF16, FLOF_ERR_ARG,
+                     "precision: FLOF_PSTORE_F32 or FLOF_PSTORE_BF16");
+        FLOF_REQUIRE(v == FLOF_PSTORE_F32 || c->use_tmarch, FLOF_ERR_ARG,
+                     "precision: reduced storage needs the t-marching kernels");
+        c->p_store = v;
+    }
+    ~PStoreScope() { c->p_store = old; }
+};
+
 static MsParams ms_params(const flof_of_params *p)
 {
     FLOF_REQUIRE(p->blurType == 1 || p->postVelBlur <= 0., FLOF_ERR_ARG, "NYI");
@@ -1588,6 +1689,7 @@ int flof_optical_flow4d(flof_grid4d *vel, flof_grid4d *i0, flof_grid4d *i1,
     }
     FLOF_REQUIRE(p, FLOF_ERR_ARG, "opticalFlow4d: null params");
     FLOF_REQUIRE(p->blurType == 1 || p->postVelBlur <= 0., FLOF_ERR_ARG, "NYI");
+    PStoreScope ps(ctx, p->precision);
     Dims4 D = dims_of(vel);
     const int64_t n = vel->n();
     // the incoming flow feeds the rhs; the solution replaces it
@@ -1639,6 +1741,7 @@ int flof_optical_flow_multiscale4d(flof_grid4d *vel, flof_grid4d *i0, flof_grid4
         need_dims(vel, warm, "opticalFlowMultiscale4d");
     }
     const MsParams P = ms_params(p);
+    PStoreScope ps(ctx, p->precision);
     flof_solve_info local;
     flof_solve_info *inf = info ? info : &local;
     std::memset(inf, 0, sizeof(*inf));
@@ -1693,6 +1796,7 @@ int flof_optical_flow_multiscale4d_batch(flof_ctx *ctx, int n, flof_grid4d *cons
             const int j = next.fetch_add(1);
             if (j >= n) break;
             try {
+                PStoreScope ps(w, p->precision);
                 flof_solve_info local;
                 flof_solve_info *inf = infos ? &infos[j] : &local;
                 std::memset(inf, 0, sizeof(*inf));

@@ -265,9 +265,11 @@ def opticalFlowMultiscale4d(vel, i0, i1, rhsT=None, wSmooth=0., wEnergy=0.,
                             blurType=1, cfl=999, orderTime=1, orderSpace=1,
                             resetBndWidth=-1., multiStep=1, projSizeThresh=9999,
                             minGridSize=10, doFinalProject=False, warm=None,
-                            cgMaxIter=1000):
-    """optflow4d.cpp:2182-2195.  `warm` (extension) is an initial guess of
-    the total flow for the CG of every level."""
+                            cgMaxIter=1000, precision=0):
+    """optflow4d.cpp:2182-2195.  Extensions: `warm` is an initial guess of
+    the total flow for the CG of every level; `precision` the storage of the
+    CG search direction (FLOF_PSTORE_F32 = reference arithmetic,
+    FLOF_PSTORE_BF16)."""
     if i0.parent.getDt() != 1.0:
         raise FlofError(-6, "Invalid, only dt 1 for now!")
     p = _params(wSmooth=wSmooth, wEnergy=wEnergy, level=level,
@@ -276,7 +278,8 @@ def opticalFlowMultiscale4d(vel, i0, i1, rhsT=None, wSmooth=0., wEnergy=0.,
                 orderSpace=orderSpace, resetBndWidth=resetBndWidth,
                 multiStep=multiStep, projSizeThresh=projSizeThresh,
                 minGridSize=minGridSize, doFinalProject=int(doFinalProject),
-                cgMaxIter=cgMaxIter, warmStart=int(warm is not None))
+                cgMaxIter=cgMaxIter, warmStart=int(warm is not None),
+                precision=precision)
     err = C.c_float()
     check(_lib.lib().flof_optical_flow_multiscale4d(
         vel.h, i0.h, i1.h, rhsT.h if rhsT is not None else None, C.byref(p),
@@ -306,12 +309,13 @@ def opticalFlowMultiscale4dBatch(vels, i0s, i1s, concurrency=4, **kw):
 
 def opticalFlow4d(vel, i0, i1, rhsT=None, wSmooth=0., wEnergy=0.,
                   postVelBlur=0., cgAccuracy=1e-04, blurType=1,
-                  resetBndWidth=-1., cgMaxIter=1000):
-    """optflow4d.cpp:2110-2119 (cgMaxIter: the reference's fixed cap of 1000,
-    optflow4d.cpp:502, exposed as an extension)."""
+                  resetBndWidth=-1., cgMaxIter=1000, precision=0):
+    """optflow4d.cpp:2110-2119 (extensions: cgMaxIter, the reference's fixed
+    cap of 1000 at optflow4d.cpp:502; precision as for the multiscale
+    solve)."""
     p = _params(wSmooth=wSmooth, wEnergy=wEnergy, postVelBlur=postVelBlur,
                 cgAccuracy=cgAccuracy, blurType=blurType,
-                resetBndWidth=resetBndWidth, cgMaxIter=cgMaxIter)
+                resetBndWidth=resetBndWidth, cgMaxIter=cgMaxIter, precision=precision)
     check(_lib.lib().flof_optical_flow4d(
         vel.h, i0.h, i1.h, rhsT.h if rhsT is not None else None, C.byref(p),
         C.byref(_LAST_INFO)), vel.ctx)

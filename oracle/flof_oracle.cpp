@@ -134,6 +134,30 @@ struct DSum {
     inline double value() const { return g_sum_mode == 0 ? hi : hi + lo; }
 };
 
+// Storage of the CG search direction (extension; no reference counterpart):
+// 0 keeps it in R, 1 rounds every stored value to bfloat16 (round to nearest
+// even, through float) and the rounded value is the one used.
+static int g_pstore = 0;
+
+// float -> bfloat16 -> float, round to nearest even (NaN stays NaN)
+static float round_bf16(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return f;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+template <class R> static inline R round_p(R v)
+{
+    if (g_pstore == 1) return (R)round_bf16((float)v);
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // Quadrilinear sampling, util/vector4d.h:422-516.  Cell centres are at c+1/2;
 // per dimension q = p - 1/2, i = (int)q, w1 = q - i; q < 0 clamps to (0; 1,0),
@@ -508,6 +532,8 @@ static CgResult pcg(const std::vector<R> &offd, const std::vector<R> &blockd,
         return out;
     }
     srch = tmp;
+    if (g_pstore)
+        for (auto &v : srch) v = round_p(v);
     for (int it = 0; it < maxIter; ++it) {
         applyMat(offd, blockd, off, srch, tmp);
         const double alpha = sigma / dotProd(srch, tmp);
@@ -525,6 +551,8 @@ static CgResult pcg(const std::vector<R> &offd, const std::vector<R> &blockd,
         const double beta = sigmaNew / sigma;
         addScaled(tmp, beta, srch);
         srch = tmp;
+        if (g_pstore)
+            for (auto &v : srch) v = round_p(v);
         sigma = sigmaNew;
     }
     out.residual = (R)(residual / resIni);
@@ -1357,6 +1385,7 @@ using namespace orc;
         const flof_of_params *p, double dt, const R *warm, int32_t *iters,     \
         double *residual, int32_t *status)                                     \
     {                                                                          \
+        g_pstore = p->precision;                                               \
         CgResult cg = opticalFlowLevel<R>(                                     \
             (V4<R> *)vel, i0, i1, rhsT, Dims{X, Y, Z, T}, (R)p->wSmooth,       \
             (R)p->wEnergy, (R)p->postVelBlur, (R)p->cgAccuracy,                \
@@ -1372,6 +1401,7 @@ using namespace orc;
     {                                                                          \
         if (info) std::memset(info, 0, sizeof(*info));                         \
         OfParams P = fromC(p);                                                 \
+        g_pstore = p->precision;                                               \
         return (double)multiscale<R>((V4<R> *)vel, i0, i1, rhsT,               \
                                      Dims{X, Y, Z, T}, P, p->level, (R)dt,     \
                                      info, (const V4<R> *)warm);               \
@@ -1442,6 +1472,7 @@ using namespace orc;
     }
 
 extern "C" void orc_set_sum_mode(int mode) { g_sum_mode = mode; }
+extern "C" float orc_round_bf16(float f) { return round_bf16(f); }
 extern "C" int orc_get_sum_mode(void) { return g_sum_mode; }
 
 ORC_DEFINE(float, _f32)

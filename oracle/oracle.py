@@ -82,6 +82,9 @@ def _declare(L):
     L.orc_set_sum_mode.restype = None
     L.orc_get_sum_mode.argtypes = []
     L.orc_get_sum_mode.restype = C.c_int
+    for name in ("orc_round_bf16",):
+        getattr(L, name).argtypes = [C.c_float]
+        getattr(L, name).restype = C.c_float
     for suf in ("_f32", "_f64"):
         sig = {
             "orc_sample4_real": ([_P, _I, _I, _I, _I, _P, _I64, _P], None),
@@ -125,6 +128,12 @@ def set_sum_mode(mode):
     """0: FP64 sums in the reference's serial order (default); 1: correctly
     rounded sums (the GPU library's order-independent summation)."""
     lib().orc_set_sum_mode(int(mode))
+
+
+def round_bf16(x):
+    """The oracle's float -> bfloat16 -> float rounding (search-direction
+    storage precision 1)."""
+    return lib().orc_round_bf16(float(x))
 
 
 def get_sum_mode():

@@ -1,0 +1,66 @@
+"""Reduced-precision storage of the CG search direction (bfloat16, an
+extension): the GPU result is bit-identical to the oracle run with the same
+storage (correctly rounded sums), and its distance to the FP64 oracle is
+bounded and reported."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ofblend_amd import manta as M
+from ofblend_amd.abi import FLOF_PSTORE_BF16, default_params
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _grids(s, a, b):
+    g0, g1 = s.create(M.Grid4Real), s.create(M.Grid4Real)
+    g0.from_numpy(a)
+    g1.from_numpy(b)
+    return g0, g1
+
+
+def test_bf16_direction_level_solve_bit_exact():
+    N, T = 14, 12
+    i0 = O.drop_sdf(N, N, N, T, seed=1)
+    i1 = O.drop_sdf(N, N, N, T, seed=2)
+    say("bf16 level solve", (N, N, N, T))
+    s = M.Solver(gridSize=M.vec3(N, N, N), fourthDim=T)
+    g0, g1 = _grids(s, i0, i1)
+    vel = s.create(M.Grid4Vec4)
+    M.opticalFlow4d(vel, g0, g1, wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5,
+                    precision=FLOF_PSTORE_BF16)
+    it_gpu = M.getLastSolveInfo().summary()["levels"][0]["iters"]
+    p = default_params(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5,
+                       precision=FLOF_PSTORE_BF16)
+    with O.sum_mode(1):
+        ref, it, res, st = O.optical_flow4d(np.zeros(i0.shape + (4,), np.float32), i0, i1, p)
+    say("iterations gpu", it_gpu, "oracle", it)
+    assert it_gpu == [it]
+    assert np.array_equal(vel.to_numpy(), ref)
+
+
+def test_bf16_direction_multiscale_bit_exact_and_error():
+    N, T = 24, 20
+    i0 = O.drop_sdf(N, N, N, T, seed=5, nspheres=2, rmin=0.08, rmax=0.18)
+    i1 = O.drop_sdf(N, N, N, T, seed=6, nspheres=2, rmin=0.08, rmax=0.18)
+    say("bf16 multiscale", (N, N, N, T))
+    s = M.Solver(gridSize=M.vec3(N, N, N), fourthDim=T)
+    g0, g1 = _grids(s, i0, i1)
+    vel = s.create(M.Grid4Vec4)
+    prm = dict(O.P_PLAIN, minGridSize=8)
+    M.opticalFlowMultiscale4d(vel=vel, i0=g0, i1=g1, precision=FLOF_PSTORE_BF16, **prm)
+    v = vel.to_numpy()
+    with O.sum_mode(1):
+        ref, _, _ = O.multiscale4d(np.zeros(v.shape, np.float32), i0, i1,
+                                   default_params(precision=FLOF_PSTORE_BF16, **prm))
+    assert np.array_equal(v, ref)
+    f64, _, _ = O.multiscale4d(np.zeros(v.shape), i0.astype(np.float64),
+                               i1.astype(np.float64), default_params(**prm))
+    d = v - f64
+    say("bf16 vs FP64 oracle: max %.3e rms %.3e (|v| max %.2f)" % (
+        np.abs(d).max(), np.sqrt((d ** 2).mean()), np.abs(f64).max()))
+    assert np.abs(d).max() < 5e-2
