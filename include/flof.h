@@ -55,6 +55,7 @@ enum flof_dtype {
 typedef struct flof_ctx flof_ctx;
 typedef struct flof_grid4d flof_grid4d;
 typedef struct flof_warp_state flof_warp_state;
+typedef struct flof_warmnet flof_warmnet;
 
 /* Parameters of opticalFlowMultiscale4d / opticalFlow4d.  Field names and
  * defaults follow the keyword arguments of
@@ -342,6 +343,35 @@ int flof_warp_state_run(flof_warp_state *s, float *dst_dev, int dx, int dy,
                         int zeroVel, int bordSkip, float defoAniFac);
 /* loadAdvectTimeSlice_Finish. */
 int flof_warp_state_close(flof_warp_state *s);
+
+/* ---- learned warm start (no reference counterpart) ---------------------- */
+/* A space-time convolutional network (lift 2->16, four residual blocks of a
+ * 3x3x3 space and a 3-tap time convolution with 16 channels, head 16->4)
+ * that predicts a 4D flow from the two SDFs; BF16 weights on the matrix
+ * cores with FP32 accumulation, FP32 master weights trained with Adam.
+ * Parameters are a flat float array of flof_warmnet_num_params() values. */
+int flof_warmnet_num_params(void);
+int flof_warmnet_create(flof_ctx *ctx, uint64_t seed, flof_warmnet **out);
+int flof_warmnet_destroy(flof_warmnet *net);
+int flof_warmnet_get_params(flof_warmnet *net, float *host, int n);
+int flof_warmnet_set_params(flof_warmnet *net, const float *host, int n);
+/* Gradient of the last flof_warmnet_loss_grad call. */
+int flof_warmnet_get_grads(flof_warmnet *net, float *host, int n);
+/* flow (VEC4, cells of the input grid) = net(i0, i1). */
+int flof_warmnet_forward(flof_warmnet *net, flof_grid4d *i0, flof_grid4d *i1,
+                         flof_grid4d *flow);
+/* Mean squared error against target (VEC4) and its gradient (kept in the
+ * net for flof_warmnet_adam / flof_warmnet_get_grads). */
+int flof_warmnet_loss_grad(flof_warmnet *net, flof_grid4d *i0, flof_grid4d *i1,
+                           flof_grid4d *target, double *loss);
+/* One Adam step with the last gradient. */
+int flof_warmnet_adam(flof_warmnet *net, float lr, float beta1, float beta2,
+                      float eps);
+/* Warm start for opticalFlowMultiscale4d: i0, i1 are restricted along the
+ * solve's pyramid to the first level with X <= netX, the net runs there and
+ * its flow is prolonged (x2 per level) into warm (VEC4, size of i0). */
+int flof_warmnet_warm_start(flof_warmnet *net, flof_grid4d *i0,
+                            flof_grid4d *i1, int netX, flof_grid4d *warm);
 
 /* ---- synthetic inputs (no reference counterpart) ----------------------- */
 /* Seeded falling-drop SDF (replaces the FLIP input simulation of

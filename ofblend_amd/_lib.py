@@ -110,6 +110,16 @@ SIGNATURES = {
     "flof_warp_state_run": ([_P, _P, _I, _I, _I, _P, _F, _F, _F, _FP, _FP, _FP, _FP,
                              _F, _I, _I, _F], _I),
     "flof_warp_state_close": ([_P], _I),
+    "flof_warmnet_num_params": ([], _I),
+    "flof_warmnet_create": ([_P, _U64, C.POINTER(_P)], _I),
+    "flof_warmnet_destroy": ([_P], _I),
+    "flof_warmnet_get_params": ([_P, _FP, _I], _I),
+    "flof_warmnet_set_params": ([_P, _FP, _I], _I),
+    "flof_warmnet_get_grads": ([_P, _FP, _I], _I),
+    "flof_warmnet_forward": ([_P, _P, _P, _P], _I),
+    "flof_warmnet_loss_grad": ([_P, _P, _P, _P, _DP], _I),
+    "flof_warmnet_adam": ([_P, _F, _F, _F, _F], _I),
+    "flof_warmnet_warm_start": ([_P, _P, _P, _I, _P], _I),
     "flof_dev_alloc": ([_P, _SZ, C.POINTER(_P)], _I),
     "flof_dev_free": ([_P, _P], _I),
     "flof_dev_upload": ([_P, _P, _P, _SZ], _I),

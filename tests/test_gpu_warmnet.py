@@ -1,0 +1,106 @@
+"""The learned warm start on the GPU: the matrix-core forward and backward
+kernels against the CPU restatement (oracle/warmnet.py), a training run that
+lowers the loss, and the warm-started solve against the plain one."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import warmnet as W
+from ofblend_amd import manta as M
+from ofblend_amd import warmstart as WS
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(8)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _case(shape=(6, 8, 8, 16), seed=3):
+    T, Z, Y, X = shape
+    s = M.Solver(gridSize=M.vec3(X, Y, Z), fourthDim=T)
+    a = O.drop_sdf(X, Y, Z, T, seed=seed).astype(np.float32)
+    b = O.drop_sdf(X, Y, Z, T, seed=seed + 1).astype(np.float32)
+    g0, g1 = s.create(M.Grid4Real), s.create(M.Grid4Real)
+    g0.from_numpy(a)
+    g1.from_numpy(b)
+    return s, a, b, g0, g1
+
+
+def test_forward_matches_oracle():
+    say("warmnet forward")
+    s, a, b, g0, g1 = _case()
+    net = WS.WarmNet(seed=5)
+    P = (RNG.standard_normal(WS.num_params()) * 0.2).astype(np.float32)
+    net.set_params(P)
+    flow = s.create(M.Grid4Vec4)
+    net.forward(g0, g1, flow)
+    y = flow.to_numpy()
+    ref = W.forward(P.astype(np.float64), a, b, rnd=True)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    say("forward max rel err %.2e" % err)
+    assert err < 2e-2
+    assert np.sqrt(((y - ref) ** 2).mean()) < 3e-3 * np.abs(ref).max()
+
+
+def test_gradients_match_oracle():
+    say("warmnet gradients")
+    s, a, b, g0, g1 = _case()
+    net = WS.WarmNet(seed=6)
+    P = (RNG.standard_normal(WS.num_params()) * 0.2).astype(np.float32)
+    net.set_params(P)
+    tgt = (RNG.standard_normal(a.shape + (4,))).astype(np.float32)
+    tg = s.create(M.Grid4Vec4)
+    tg.from_numpy(tgt)
+    loss = net.loss_grad(g0, g1, tg)
+    g = net.get_grads().astype(np.float64)
+    rl, rg = W.loss_grad(P.astype(np.float64), a, b, tgt, rnd=True)
+    say("loss gpu %.6f oracle %.6f" % (loss, rl))
+    assert abs(loss - rl) <= 1e-3 * rl
+    groups = {"Wl": slice(W.OFF_WL, W.OFF_BL), "bl": slice(W.OFF_BL, W.OFF_BLK),
+              "Wh": slice(W.OFF_WH, W.OFF_BH), "bh": slice(W.OFF_BH, W.NPARAM)}
+    for bk in range(W.NB):
+        o = W.OFF_BLK + bk * W.BLK_SIZE
+        groups["Ws%d" % bk] = slice(o + W.BLK_WS, o + W.BLK_BS)
+        groups["bs%d" % bk] = slice(o + W.BLK_BS, o + W.BLK_WT)
+        groups["Wt%d" % bk] = slice(o + W.BLK_WT, o + W.BLK_BT)
+        groups["bt%d" % bk] = slice(o + W.BLK_BT, o + W.BLK_SIZE)
+    for name, sl in groups.items():
+        rel = np.linalg.norm(g[sl] - rg[sl]) / max(1e-30, np.linalg.norm(rg[sl]))
+        say("grad %-4s rel err %.2e" % (name, rel))
+        assert rel < 3e-2, name
+
+
+def test_training_lowers_loss_and_warm_start_is_used():
+    """Train on plain-solve flows of 16^4 drop pairs, then warm-start unseen
+    pairs: the loss falls, the warm start is kept on the finest level, the
+    solve reaches the same residual target and its flow stays as close to
+    the plain solve's as the solve tolerance allows."""
+    say("warmnet training")
+    s, data = WS.generate(24, 16, 16, seed0=500, minGridSize=4)
+    net = WS.WarmNet(seed=7)
+    losses = WS.train(net, data, 240, lr=3e-3, seed=1, log=say)
+    first, last = np.mean(losses[:20]), np.mean(losses[-20:])
+    say("loss first %.5f last %.5f" % (first, last))
+    assert last < 0.5 * first
+    gains = []
+    for k in range(3):
+        i0, i1 = WS.drop_pair(s, 9000 + 2 * k, nspheres=1 + k % 3, rmin=0.08, rmax=0.18)
+        prm = dict(WS.PLAIN, minGridSize=4)
+        v0 = s.create(M.Grid4Vec4)
+        e0 = M.opticalFlowMultiscale4d(vel=v0, i0=i0, i1=i1, **prm)
+        it0 = M.getLastSolveInfo().summary()
+        warm = s.create(M.Grid4Vec4)
+        net.warm_start(i0, i1, warm, netX=16)
+        v1 = s.create(M.Grid4Vec4)
+        e1 = M.opticalFlowMultiscale4d(vel=v1, i0=i0, i1=i1, warm=warm, **prm)
+        it1 = M.getLastSolveInfo().summary()
+        a0 = [l["iters"][0] for l in it0["levels"]]
+        a1 = [l["iters"][0] for l in it1["levels"]]
+        used = [l["warm_used"] for l in it1["levels"]]
+        say("pair %d iters plain %s warm %s used %s err %.3f / %.3f" % (k, a0, a1, used, e0, e1))
+        assert all(r <= 1e-5 for l in it1["levels"] for r in l["residual"])
+        gains.append(sum(a0) / max(1, sum(a1)))
+    say("iteration ratio plain/warm per pair: %s" % ["%.2f" % g for g in gains])
