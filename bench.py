@@ -66,7 +66,8 @@ def main():
         os.environ["FLOF_NO_GRAPH"] = "1"  # read when the context is created
     from ofblend_amd import _lib
     from ofblend_amd import manta as M
-    from ofblend_amd.abi import FLOF_KT_CG_APPLY, FLOF_KT_CG_DIR, FLOF_KT_CG_UPDATE, FLOF_KT_WARP
+    from ofblend_amd.abi import (FLOF_KT_CG_APPLY, FLOF_KT_CG_DIR, FLOF_KT_CG_UPDATE,
+                                 FLOF_KT_WARP, FLOF_PSTORE_BF16)
 
     L = _lib.lib()
     s = M.Solver(gridSize=M.vec3(N, N, N), fourthDim=N)
@@ -110,9 +111,12 @@ def main():
         infos.append(M.getLastSolveInfo().summary())
         log("bench: step %d %.3f s iters %s" % (
             st, times[-1], [l["iters"][0] for l in infos[-1]["levels"]]))
-    ktime = {}
+    cells0 = N ** 4
+    ktime, ktime0 = {}, {}
     for k in kinds:
         n, ms = C.c_int64(), C.c_double()
+        L.flof_ctx_timing_tag(ctx, k, cells0, C.byref(n), C.byref(ms))
+        ktime0[k] = (n.value, ms.value)
         L.flof_ctx_timing(ctx, k, C.byref(n), C.byref(ms))
         L.flof_ctx_enable_timing(ctx, k, 0)
         ktime[k] = (n.value, ms.value)
@@ -133,26 +137,42 @@ def main():
         M.sync()
         t_plain.append(time.perf_counter() - t0)
 
-    # roofline of the dominant kernel: algorithmic bytes of all real launches
-    # over the summed event time of every launch of that kernel
+    # all levels: algorithmic bytes of all real launches over the summed
+    # event time of every launch of that kernel
     def kstat(kind, bpc):
         n, ms = ktime[kind]
         byts = bpc * cell_iters * args.steps
         return {"launches": n, "ms_total": ms, "avg_us": 1000.0 * ms / max(1, n),
                 "GBps": byts / (ms * 1e-3) / 1e9 if ms > 0 else None}
-    kst = {"cg_update_tm": kstat(FLOF_KT_CG_UPDATE, B_UPDATE),
-           "cg_pq_tm": kstat(FLOF_KT_CG_APPLY, B_APPLY),
-           "cg_dir_tm": kstat(FLOF_KT_CG_DIR, B_DIR)}
-    dom = max(kst, key=lambda k: kst[k]["ms_total"])
-    bpc = {"cg_update_tm": B_UPDATE, "cg_pq_tm": B_APPLY, "cg_dir_tm": B_DIR}[dom]
-    achieved = kst[dom]["GBps"] * 1e9
-    iter_ms = sum(kst[k]["ms_total"] for k in kst) / args.steps
+
+    # finest level only: launches tagged with its cell count; the iterations
+    # after convergence that the chunked host loop enqueues return at once
+    # and are counted in the time, not in the bytes
+    def kstat0(kind, bpc):
+        n, ms = ktime0[kind]
+        real = iters[0] * args.steps
+        byts = bpc * cells0 * real
+        return {"launches": n, "iterations": real, "ms_total": ms,
+                "avg_us_per_iteration": 1000.0 * ms / max(1, real),
+                "bytes_per_launch": bpc * cells0,
+                "GBps": byts / (ms * 1e-3) / 1e9 if ms > 0 else None}
+    names = {FLOF_KT_CG_UPDATE: ("k_cg_update_tm", B_UPDATE),
+             FLOF_KT_CG_APPLY: ("k_cg_pq_tm", B_APPLY), FLOF_KT_CG_DIR: ("k_cg_dir_tm", B_DIR)}
+    kst = {names[k][0]: kstat(k, names[k][1]) for k in kinds}
+    kst0 = {names[k][0]: kstat0(k, names[k][1]) for k in kinds}
+    dom = max(kst0, key=lambda k: kst0[k]["ms_total"])
+    achieved = kst0[dom]["GBps"] * 1e9
+    iter_ms0 = sum(kst0[k]["ms_total"] for k in kst0) / (iters[0] * args.steps)
+    traffic = committed_traffic(dom)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                "kernel": "k_" + dom,
-                "bytes_per_cell": bpc,
-                "cells_per_launch_avg": cell_iters / max(1, sum(iters)),
-                "pcg_iteration_frac_112B": (B_ITER_MIN * cell_iters / (iter_ms * 1e-3)) / HBM_PEAK}
+                "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                "traffic": traffic,
+                "kernel": dom, "launch": "finest level, %d cells" % cells0,
+                "bytes_per_cell": {"k_cg_update_tm": B_UPDATE, "k_cg_pq_tm": B_APPLY,
+                                   "k_cg_dir_tm": B_DIR}[dom],
+                "avg_launch_us": kst0[dom]["avg_us_per_iteration"],
+                "pcg_iteration_us": 1000.0 * iter_ms0,
+                "pcg_iteration_frac_112B": (B_ITER_MIN * cells0 / (iter_ms0 * 1e-3)) / HBM_PEAK}
 
     out = {
         "metric": METRIC, "value": solves_per_s, "unit": "solves/s", "n_gpus": 1,
@@ -166,7 +186,8 @@ def main():
                    "iters_per_level": iters, "final_err": info["err_final"],
                    "residual_per_level": [l["residual"][0] for l in levels]},
         "roofline": roofline,
-        "kernels": kst,
+        "kernels_finest_level": kst0,
+        "kernels_all_levels": kst,
         "ms_per_step_without_kernel_events": 1000.0 * min(t_plain),
     }
 
@@ -183,6 +204,27 @@ def main():
     import hashlib
     out["reference_cap_1000"]["sha256"] = hashlib.sha256(vcap.tobytes()).hexdigest()
 
+    # the search direction stored as bfloat16 (extension, not the headline:
+    # the reference computes in FP32); second of two solves is timed
+    vel.clear()
+    solve(FIXED_RESIDUAL_CAP)
+    vf = vel.to_numpy()
+    for _ in range(2):
+        vel.clear()
+        M.sync()
+        t0 = time.perf_counter()
+        eb = M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, cgMaxIter=FIXED_RESIDUAL_CAP,
+                                       precision=FLOF_PSTORE_BF16, **PARAMS)
+        M.sync()
+        tb = time.perf_counter() - t0
+    bi = M.getLastSolveInfo().summary()
+    out["bf16_direction"] = {
+        "ms": 1000.0 * tb, "iters_per_level": [l["iters"][0] for l in bi["levels"]],
+        "final_err": eb, "final_err_fp32": info["err_final"],
+        "max_abs_diff_vs_fp32": float(np.abs(vel.to_numpy() - vf).max()),
+        "note": "x, r, g and all arithmetic FP32, CG scalars FP64; p stored as bfloat16"}
+    out["warm_start"] = warm_start(M, s, vel, i0, i1, vf, iters, ms_step)
+
     if not args.no_frames:
         out["frame_warp"] = frame_warp(M, L, ctx, s, vel, FLOF_KT_WARP)
 
@@ -190,6 +232,56 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.cpu_iters, iters, cells, ms_step)
         out["speedup_vs_cpu"] = out["cpu_baseline"]["s_per_solve"] / (ms_step / 1000.0)
     print(json.dumps(out), flush=True)
+
+
+def committed_traffic(kernel):
+    """HBM bytes per finest-level launch of `kernel` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated by a copy kernel),
+    or None if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "pcg_kernels_64x4_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("machine-readable:"):
+            d = json.loads(line.split(":", 1)[1]).get(kernel)
+            if d:
+                return d["fetch_bytes"] + d["write_bytes"]
+    return None
+
+
+def warm_start(M, s, vel, i0, i1, v_plain, iters_plain, ms_plain):
+    """The solve warm-started by the trained network (weights committed in
+    ofblend_amd/data) against the plain solve: iterations, time and the
+    SDF-alignment error of both."""
+    from ofblend_amd import warmstart as WS
+    if not os.path.exists(WS.DEFAULT_WEIGHTS):
+        return None
+    net = WS.WarmNet(seed=1)
+    net.load(WS.DEFAULT_WEIGHTS)
+    warm = s.create(M.Grid4Vec4)
+    res = {}
+    for rep in range(2):
+        M.sync()
+        t0 = time.perf_counter()
+        net.warm_start(i0, i1, warm, netX=32)
+        vel.clear()
+        err = M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, warm=warm,
+                                        cgMaxIter=FIXED_RESIDUAL_CAP, **PARAMS)
+        M.sync()
+        t = time.perf_counter() - t0
+    wi = M.getLastSolveInfo().summary()
+    vw = vel.to_numpy()
+    vel.clear()
+    e_plain = M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, cgMaxIter=FIXED_RESIDUAL_CAP,
+                                        **PARAMS)
+    it_w = [l["iters"][0] for l in wi["levels"]]
+    return {"ms_including_net": 1000.0 * t, "ms_plain": ms_plain,
+            "iters_per_level": it_w, "iters_per_level_plain": iters_plain,
+            "finest_level_iteration_ratio": iters_plain[0] / max(1, it_w[0]),
+            "warm_kept_per_level": [l["warm_used"] for l in wi["levels"]],
+            "final_err": err, "final_err_plain": e_plain,
+            "max_abs_diff_vs_plain": float(np.abs(vw - v_plain).max()),
+            "net_level": 32}
 
 
 def frame_warp(M, L, ctx, s, vel, kind):

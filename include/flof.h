@@ -126,8 +126,8 @@ int flof_ctx_trim(flof_ctx *ctx);
  * every launch of kernel class `kind` (FLOF_KT_*) is bracketed by events;
  * flof_ctx_timing returns the number of launches and their summed duration. */
 enum flof_kernel_kind {
-    FLOF_KT_CG_APPLY = 0,   /* q = A p and p.q                         */
-    FLOF_KT_CG_UPDATE = 1,  /* x,r update, signed max, z.r             */
+    FLOF_KT_CG_APPLY = 0,   /* p.(A p)                                 */
+    FLOF_KT_CG_UPDATE = 1,  /* x, r update, signed max of r, z.r       */
     FLOF_KT_CG_DIR = 2,     /* p = z + beta p                          */
     FLOF_KT_WARP = 3,       /* 3D frame warp                           */
     FLOF_KT_ADVECT = 4,     /* 4D semi-Lagrangian advection            */
@@ -135,6 +135,11 @@ enum flof_kernel_kind {
 };
 int flof_ctx_enable_timing(flof_ctx *ctx, int kind, int enable);
 int flof_ctx_timing(flof_ctx *ctx, int kind, int64_t *launches, double *ms_total);
+/* The same for the launches of one grid size only, without clearing the
+ * records (flof_ctx_timing clears them): PCG launches are tagged with the
+ * cell count of their level (X*Y*Z*T); tag < 0 takes all. */
+int flof_ctx_timing_tag(flof_ctx *ctx, int kind, int64_t tag, int64_t *launches,
+                        double *ms_total);
 
 /* Autotuner of the PCG kernels' launch shape (no reference counterpart):
  * while enabled, the first solve of each grid shape times the candidate

@@ -875,8 +875,8 @@ int flof_ctx_destroy(flof_ctx *ctx)
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &t : ctx->timers)
         for (auto &ev : t.pending) {
-            (void)hipEventDestroy(ev.first);
-            (void)hipEventDestroy(ev.second);
+            (void)hipEventDestroy(ev.e0);
+            (void)hipEventDestroy(ev.e1);
         }
     ctx->pool.trim();
     if (ctx->red) (void)hipFree(ctx->red);
@@ -934,27 +934,38 @@ int flof_ctx_enable_timing(flof_ctx *ctx, int kind, int enable)
     API_CATCH(ctx)
 }
 
-int flof_ctx_timing(flof_ctx *ctx, int kind, int64_t *launches, double *ms_total)
+int flof_ctx_timing_tag(flof_ctx *ctx, int kind, int64_t tag, int64_t *launches,
+                        double *ms_total)
 {
     API_TRY
     FLOF_REQUIRE(ctx && kind >= 0 && kind < FLOF_KT_COUNT, FLOF_ERR_ARG,
                  "flof_ctx_timing: bad kind");
     FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    int64_t n = 0;
+    double ms_sum = 0.0;
+    for (auto &ev : ctx->timers[kind].pending) {
+        if (tag >= 0 && ev.tag != tag) continue;
+        float ms = 0.f;
+        FLOF_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
+        ms_sum += ms;
+        n += 1;
+    }
+    if (launches) *launches = n;
+    if (ms_total) *ms_total = ms_sum;
+    API_CATCH(ctx)
+}
+
+int flof_ctx_timing(flof_ctx *ctx, int kind, int64_t *launches, double *ms_total)
+{
+    const int rc = flof_ctx_timing_tag(ctx, kind, -1, launches, ms_total);
+    if (rc != FLOF_OK) return rc;
     KTimer &t = ctx->timers[kind];
     for (auto &ev : t.pending) {
-        float ms = 0.f;
-        FLOF_HIP(hipEventElapsedTime(&ms, ev.first, ev.second));
-        t.ms += ms;
-        t.launches += 1;
-        (void)hipEventDestroy(ev.first);
-        (void)hipEventDestroy(ev.second);
+        (void)hipEventDestroy(ev.e0);
+        (void)hipEventDestroy(ev.e1);
     }
     t.pending.clear();
-    if (launches) *launches = t.launches;
-    if (ms_total) *ms_total = t.ms;
-    t.launches = 0;
-    t.ms = 0.0;
-    API_CATCH(ctx)
+    return FLOF_OK;
 }
 
 int flof_grid4d_create(flof_ctx *ctx, int X, int Y, int Z, int T, int elem, int dtype,

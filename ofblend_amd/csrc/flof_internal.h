@@ -58,9 +58,11 @@ struct Pool {
 
 struct KTimer {
     bool on = false;
-    int64_t launches = 0;
-    double ms = 0.0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    struct Rec {
+        hipEvent_t e0, e1;
+        int64_t tag;  // cells of the grid the launch worked on
+    };
+    std::vector<Rec> pending;
 };
 
 }  // namespace flof
@@ -85,6 +87,8 @@ struct flof_ctx {
     // device scratch for reductions (partials + state), grown on demand
     void *red = nullptr;
     size_t red_bytes = 0;
+    // tag recorded with timed launches (the PCG sets the level's cell count)
+    int64_t timer_tag = 0;
     // worker contexts of batched solves (same device, own streams)
     std::vector<flof_ctx *> workers;
     // t-march autotuner: on/off and the chosen workgroups per CU per shape
@@ -159,7 +163,7 @@ inline void klaunch(flof_ctx *ctx, int kind, void (*kernel)(KArgs...), dim3 grid
         FLOF_HIP(hipEventCreate(&e0));
         FLOF_HIP(hipEventCreate(&e1));
         hipExtLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, e0, e1, 0, args...);
-        ctx->timers[kind].pending.emplace_back(e0, e1);
+        ctx->timers[kind].pending.push_back({e0, e1, ctx->timer_tag});
     } else {
         hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, args...);
     }

@@ -33,6 +33,11 @@
 
 #include "flof_internal.h"
 
+// unroll factor of the t-marching loops
+#ifndef FLOF_TM_UNROLL
+#define FLOF_TM_UNROLL 1
+#endif
+
 namespace flof {
 
 struct CgState {
@@ -683,6 +688,7 @@ k_cg_pq_tm(const typename PS::T *__restrict__ p, const float *__restrict__ i1, L
         float4 pc = PS::ld(p, c3 + D.st * w.t0);
         float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
         float ic = i1[c3 + D.st * w.t0];
+#pragma unroll FLOF_TM_UNROLL
         for (int t = w.t0; t < w.t1; ++t) {
             const int64_t c = c3 + D.st * t;
             const bool tn = t + 1 < D.T;
@@ -725,6 +731,7 @@ k_cg_update_tm(float4 *__restrict__ r, float4 *__restrict__ x,
         float4 pc = PS::ld(p, c3 + D.st * w.t0);
         float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
         float ic = i1[c3 + D.st * w.t0];
+#pragma unroll FLOF_TM_UNROLL
         for (int t = w.t0; t < w.t1; ++t) {
             const int64_t c = c3 + D.st * t;
             const bool tn = t + 1 < D.T;
@@ -784,6 +791,7 @@ k_cg_dir_tm(typename PS::T *__restrict__ p, const float4 *__restrict__ r,
     const int64_t c3 = w.c3;
     float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
     float ic = i1[c3 + D.st * w.t0];
+#pragma unroll FLOF_TM_UNROLL
     for (int t = w.t0; t < w.t1; ++t) {
         const int64_t c = c3 + D.st * t;
         const float ip = t + 1 < D.T ? i1[c + D.st] : 0.f;
@@ -1300,6 +1308,7 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
     // Kernels of iterations after convergence (or the cap, counted on the
     // device) return at once.
     const int CHUNK = 16;
+    ctx->timer_tag = n;  // timed launches of this level carry its cell count
     bool timed = false;
     for (int k : {FLOF_KT_CG_APPLY, FLOF_KT_CG_UPDATE, FLOF_KT_CG_DIR})
         timed = timed || ctx->timers[k].on;

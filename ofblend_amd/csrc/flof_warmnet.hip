@@ -12,6 +12,8 @@
 //   4 blocks:  s = Ws * h + bs                  3x3x3 space conv 16 -> 16
 //              a = relu(s)
 //              h = h + (Wt * a + bt)            3-tap time conv 16 -> 16
+//   block b spaces its taps 2^b cells apart (dilation 1, 2, 4, 8), so the
+//   network sees 31 cells in x, y, z and t: as far as the flows reach
 //   flow = head(h)                              pointwise 16 -> 4
 // The convolutions run on the matrix cores: v_mfma_f32_16x16x32_bf16 with
 // BF16 weights and BF16-rounded activations, FP32 accumulation.  A wavefront
@@ -85,7 +87,8 @@ __device__ __forceinline__ bf16x8 pack8(const float4 &a, const float4 &b)
 
 struct Geo {
     int X, Y, Z, T;
-    int n;  // cells (< 2^31, checked by the host)
+    int n;    // cells (< 2^31, checked by the host)
+    int dil;  // tap spacing of the convolutions
 };
 
 __device__ __forceinline__ void cell_coords(const Geo &g, int c, int &i, int &j, int &k, int &t)
@@ -127,6 +130,7 @@ k_conv(const float *__restrict__ in, float *__restrict__ out, float *__restrict_
        int flags)
 {
     constexpr int NT = SPACE ? NS : NTT;
+    const int dil = g.dil;
     constexpr int NP = (NT + 1) / 2;
     const int lane = threadIdx.x & 63;
     const int grp = lane >> 4;       // 0..3
@@ -148,8 +152,9 @@ k_conv(const float *__restrict__ in, float *__restrict__ out, float *__restrict_
             if (live && m < NT) {
                 int dx, dy, dz, dt;
                 tap_offset(SPACE, m, dx, dy, dz, dt);
-                const int ni = i + sgn * dx, nj = j + sgn * dy, nk = k + sgn * dz,
-                          nt = t + sgn * dt;
+                const int sd = sgn * dil;
+                const int ni = i + sd * dx, nj = j + sd * dy, nk = k + sd * dz,
+                          nt = t + sd * dt;
                 if (ni >= 0 && nj >= 0 && nk >= 0 && nt >= 0 && ni < g.X && nj < g.Y &&
                     nk < g.Z && nt < g.T) {
                     const int nc = ni + g.X * (nj + g.Y * (nk + g.Z * nt));
@@ -241,7 +246,8 @@ k_conv_wgrad(const float *__restrict__ in, const float *__restrict__ dout,
             float av[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int ni = ci[q] + dx, nj = cj[q] + dy, nk = ck[q] + dz, nt = ct[q] + dt;
+                const int ni = ci[q] + g.dil * dx, nj = cj[q] + g.dil * dy,
+                          nk = ck[q] + g.dil * dz, nt = ct[q] + g.dil * dt;
                 av[q] = (live[q] && ni >= 0 && nj >= 0 && nk >= 0 && nt >= 0 && ni < g.X &&
                          nj < g.Y && nk < g.Z && nt < g.T)
                             ? in[(int64_t)(ni + g.X * (nj + g.Y * (nk + g.Z * nt))) * C + ch]
@@ -494,7 +500,7 @@ void pack_all(flof_warmnet *net)
 Geo geo_of(const flof_grid4d *g)
 {
     FLOF_REQUIRE(g->n() < (1ll << 31) / C, FLOF_ERR_DIM, "warm-start net: grid too large");
-    return Geo{g->X, g->Y, g->Z, g->T, (int)g->n()};
+    return Geo{g->X, g->Y, g->Z, g->T, (int)g->n(), 1};
 }
 
 int conv_blocks(flof_ctx *ctx, int n)
@@ -518,9 +524,10 @@ struct Acts {
     Tmp h[NB + 1], s[NB], a[NB];
 };
 
-void forward(flof_warmnet *net, const float *i0, const float *i1, const Geo &g, Acts &A,
+void forward(flof_warmnet *net, const float *i0, const float *i1, const Geo &g0, Acts &A,
              float4 *flow)
 {
+    const Geo &g = g0;
     flof_ctx *ctx = net->ctx;
     if (!net->frag_ok) pack_all(net);
     const size_t ab = (size_t)g.n * C * 4;
@@ -535,6 +542,8 @@ void forward(flof_warmnet *net, const float *i0, const float *i1, const Geo &g, 
     for (int b = 0; b < NB; ++b) {
         const float *blk = net->P + OFF_BLK + b * BLK_SIZE;
         const bf16x8 *f = net->frag + (size_t)b * FRAG_BLK;
+        Geo g = g0;
+        g.dil = 1 << b;
         conv<true>(ctx, A.h[b].as<float>(), A.s[b].as<float>(), A.a[b].as<float>(), f,
                    blk + BLK_BS, nullptr, nullptr, g, 1, EP_BIAS);
         conv<false>(ctx, A.a[b].as<float>(), A.h[b + 1].as<float>(), nullptr, f + 2 * FRAG_S,
@@ -593,8 +602,9 @@ void conv_wgrad(flof_ctx *ctx, const float *in, const float *dout, const Geo &g,
 
 // forward + loss + backward; gradients into net->G, returns the loss
 double loss_grad(flof_warmnet *net, const float *i0, const float *i1, const float4 *tgt,
-                 const Geo &g)
+                 const Geo &g0)
 {
+    const Geo &g = g0;
     flof_ctx *ctx = net->ctx;
     Acts A;
     Tmp flow(ctx, (size_t)g.n * 16), dy(ctx, (size_t)g.n * 16);
@@ -616,6 +626,8 @@ double loss_grad(flof_warmnet *net, const float *i0, const float *i1, const floa
     hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, ctx->stream,
                        (const float *)hp.as<float>(), hb, C * 4 + 4, net->G + OFF_WH, 0);
     for (int b = NB - 1; b >= 0; --b) {
+        Geo g = g0;
+        g.dil = 1 << b;
         float *gb = net->G + OFF_BLK + b * BLK_SIZE;
         const float *blk = net->P + OFF_BLK + b * BLK_SIZE;
         const bf16x8 *f = net->frag + (size_t)b * FRAG_BLK;
@@ -849,7 +861,7 @@ int flof_warmnet_warm_start(flof_warmnet *net, flof_grid4d *i0, flof_grid4d *i1,
     }
     const Dims4 &Dn = dims[L - 1];
     FLOF_REQUIRE(Dn.st * Dn.T < (1ll << 31) / C, FLOF_ERR_DIM, "warmnet: grid too large");
-    const Geo g{Dn.X, Dn.Y, Dn.Z, Dn.T, (int)(Dn.st * Dn.T)};
+    const Geo g{Dn.X, Dn.Y, Dn.Z, Dn.T, (int)(Dn.st * Dn.T), 1};
     float4 *out = L == 1 ? (float4 *)warm->data : nullptr;
     if (L > 1) {
         v[L - 1] = Tmp(ctx, (size_t)g.n * 16);

@@ -14,7 +14,8 @@ Parameter layout (flat float32, identical to the library):
   Wl[2][16], bl[16], then per block Ws[27][16][16] (tap, in, out), bs[16],
   Wt[3][16][16], bt[16], then Wh[16][4], bh[4].
 Space tap m: (dx, dy, dz) = (m % 3 - 1, (m // 3) % 3 - 1, m // 9 - 1);
-time tap m: dt = m - 1.  Zero padding outside the grid.
+time tap m: dt = m - 1; block b multiplies the offsets by 2^b (dilation).
+Zero padding outside the grid.
 """
 import numpy as np
 
@@ -73,16 +74,19 @@ def pack(d):
     return P
 
 
-def taps(space):
+def taps(space, dil=1):
     """Offsets (dt, dz, dy, dx) of the taps in tap order."""
     if space:
-        return [(0, m // 9 - 1, (m // 3) % 3 - 1, m % 3 - 1) for m in range(NS)]
-    return [(m - 1, 0, 0, 0) for m in range(NT)]
+        return [(0, dil * (m // 9 - 1), dil * ((m // 3) % 3 - 1), dil * (m % 3 - 1))
+                for m in range(NS)]
+    return [(dil * (m - 1), 0, 0, 0) for m in range(NT)]
 
 
 def shift(a, off):
     """s[c] = a[c + off] with zeros outside; a is (T, Z, Y, X, ch)."""
     out = np.zeros_like(a)
+    if any(abs(d) >= n for d, n in zip(off, a.shape[:4])):
+        return out
     src = []
     dst = []
     for d, n in zip(off, a.shape[:4]):
@@ -96,24 +100,24 @@ def shift(a, off):
     return out
 
 
-def conv(x, W, space, sgn=1, rnd=True):
+def conv(x, W, space, sgn=1, rnd=True, dil=1):
     """out[c] = sum_tap x[c + sgn*off(tap)] @ W[tap] (W[tap] is in x out)."""
     xr = bf16(x) if rnd else x
     Wr = bf16(W) if rnd else W
     out = np.zeros(x.shape[:4] + (W.shape[2],))
-    for m, off in enumerate(taps(space)):
+    for m, off in enumerate(taps(space, dil)):
         out += shift(xr, tuple(sgn * o for o in off)) @ Wr[m]
     return out
 
 
-def wgrad(x, dout, space, rnd=True):
+def wgrad(x, dout, space, rnd=True, dil=1):
     """dW[tap] = sum_c x[c + off(tap)]^T dout[c]."""
     xr = bf16(x) if rnd else x
     dr = bf16(dout) if rnd else dout
     n = x.shape[4]
     g = np.zeros((NS if space else NT, n, dout.shape[4]))
     d2 = dr.reshape(-1, dr.shape[4])
-    for m, off in enumerate(taps(space)):
+    for m, off in enumerate(taps(space, dil)):
         g[m] = shift(xr, off).reshape(-1, n).T @ d2
     return g
 
@@ -124,10 +128,10 @@ def forward(P, i0, i1, rnd=True, keep=False):
     x = np.stack([np.asarray(i0, np.float64) * 0.25, np.asarray(i1, np.float64) * 0.25], -1)
     h = x @ d["Wl"] + d["bl"]
     hs, ss, as_ = [h], [], []
-    for blk in d["blocks"]:
-        s = conv(h, blk["Ws"], True, rnd=rnd) + blk["bs"]
+    for b, blk in enumerate(d["blocks"]):
+        s = conv(h, blk["Ws"], True, rnd=rnd, dil=2 ** b) + blk["bs"]
         a = np.maximum(s, 0.0)
-        h = h + (conv(a, blk["Wt"], False, rnd=rnd) + blk["bt"])
+        h = h + (conv(a, blk["Wt"], False, rnd=rnd, dil=2 ** b) + blk["bt"])
         ss.append(s)
         as_.append(a)
         hs.append(h)
@@ -151,14 +155,15 @@ def loss_grad(P, i0, i1, tgt, rnd=True):
     for b in range(NB - 1, -1, -1):
         blk = d["blocks"][b]
         gb = {}
-        gb["Wt"] = wgrad(as_[b], dh, False, rnd=rnd)
+        dil = 2 ** b
+        gb["Wt"] = wgrad(as_[b], dh, False, rnd=rnd, dil=dil)
         gb["bt"] = dh.reshape(-1, C).sum(0)
         Wt_T = np.transpose(blk["Wt"], (0, 2, 1))
-        ds = conv(dh, Wt_T, False, sgn=-1, rnd=rnd) * (ss[b] > 0)
-        gb["Ws"] = wgrad(hs[b], ds, True, rnd=rnd)
+        ds = conv(dh, Wt_T, False, sgn=-1, rnd=rnd, dil=dil) * (ss[b] > 0)
+        gb["Ws"] = wgrad(hs[b], ds, True, rnd=rnd, dil=dil)
         gb["bs"] = ds.reshape(-1, C).sum(0)
         Ws_T = np.transpose(blk["Ws"], (0, 2, 1))
-        dh = dh + conv(ds, Ws_T, True, sgn=-1, rnd=rnd)
+        dh = dh + conv(ds, Ws_T, True, sgn=-1, rnd=rnd, dil=dil)
         g["blocks"][b] = gb
     g["Wl"] = x.reshape(-1, 2).T @ dh.reshape(-1, C)
     g["bl"] = dh.reshape(-1, C).sum(0)

@@ -81,10 +81,15 @@ def test_training_lowers_loss_and_warm_start_is_used():
     say("warmnet training")
     s, data = WS.generate(24, 16, 16, seed0=500, minGridSize=4)
     net = WS.WarmNet(seed=7)
-    losses = WS.train(net, data, 240, lr=3e-3, seed=1, log=say)
-    first, last = np.mean(losses[:20]), np.mean(losses[-20:])
-    say("loss first %.5f last %.5f" % (first, last))
-    assert last < 0.5 * first
+
+    def fixed_loss():
+        return np.mean([net.loss_grad(*data[k]) for k in range(8)])
+
+    before = fixed_loss()
+    WS.train(net, data, 400, lr=2e-3, seed=1, log=say)
+    after = fixed_loss()
+    say("loss on 8 training pairs: before %.5f after %.5f" % (before, after))
+    assert after < 0.6 * before
     gains = []
     for k in range(3):
         i0, i1 = WS.drop_pair(s, 9000 + 2 * k, nspheres=1 + k % 3, rmin=0.08, rmax=0.18)

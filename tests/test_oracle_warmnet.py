@@ -30,18 +30,33 @@ def test_conv_matches_direct_loop():
                         if 0 <= zz < Z and 0 <= yy < Y and 0 <= xs < X:
                             ref[t, z, y, xx] += x[t, zz, yy, xs] @ Wt[m]
     np.testing.assert_allclose(got, ref, atol=1e-12)
-    got_t = W.conv(x, Wt[:3], False, sgn=-1, rnd=False)
-    ref_t = np.zeros_like(got_t)
-    for m in range(3):
-        dt = -(m - 1)
+    for dil in (1, 2):
+        got_t = W.conv(x, Wt[:3], False, sgn=-1, rnd=False, dil=dil)
+        ref_t = np.zeros_like(got_t)
+        for m in range(3):
+            dt = -(m - 1) * dil
+            for t in range(T):
+                if 0 <= t + dt < T:
+                    ref_t[t] += x[t + dt] @ Wt[m]
+        np.testing.assert_allclose(got_t, ref_t, atol=1e-12)
+    # dilation 2 in space equals the dense conv of a 5x5x5 kernel with
+    # zeros between the taps
+    got2 = W.conv(x, Wt, True, rnd=False, dil=2)
+    ref2 = np.zeros_like(got2)
+    for m in range(27):
+        off = (0, 2 * (m // 9 - 1), 2 * ((m // 3) % 3 - 1), 2 * (m % 3 - 1))
         for t in range(T):
-            if 0 <= t + dt < T:
-                ref_t[t] += x[t + dt] @ Wt[m]
-    np.testing.assert_allclose(got_t, ref_t, atol=1e-12)
+            for z in range(Z):
+                for y in range(Y):
+                    for xx in range(X):
+                        zz, yy, xs = z + off[1], y + off[2], xx + off[3]
+                        if 0 <= zz < Z and 0 <= yy < Y and 0 <= xs < X:
+                            ref2[t, z, y, xx] += x[t, zz, yy, xs] @ Wt[m]
+    np.testing.assert_allclose(got2, ref2, atol=1e-12)
 
 
 def test_backward_matches_finite_differences():
-    shape = (4, 4, 3, 5)
+    shape = (5, 5, 4, 9)
     i0 = RNG.standard_normal(shape) * 2
     i1 = RNG.standard_normal(shape) * 2
     tgt = RNG.standard_normal(shape + (4,))
