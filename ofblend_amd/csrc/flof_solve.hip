@@ -48,6 +48,7 @@ struct CgState {
     double resIni;   // signed max of the initial residual
     double resRel;   // (float)(residual / resIni), as ret_residual
     double r2, b2;   // warm-start residual check
+    double gamma;    // warm-start scale
     int iters;
     int done;
     int status;  // 0 converged, 1 not converged, 2 sigma 0/NaN, 3 tiny rhs
@@ -259,7 +260,8 @@ __global__ void k_setup_fin(CgState *st, const DD *pd, const float *pm, int nb,
     }
 }
 
-// --- warm start (extension): r_w = b - A x0, ||r_w||^2 and ||b||^2 --------
+// --- warm start (extension): guess w, scaled by the energy-optimal gamma;
+// kept if ||b - gamma A w||_2 <= ||b||_2 ------------------------------------
 __global__ void k_warm_guess(const float4 *__restrict__ warm,
                              const float4 *__restrict__ vin_acc, float4 *__restrict__ x0s,
                              Dims4 D, Coord4 C, float mDx)
@@ -279,49 +281,103 @@ __global__ void k_warm_guess(const float4 *__restrict__ warm,
     }
 }
 
+// A w, and the partial sums of b.w, w.A w and b.b (FP64 products of the FP32
+// values, correctly rounded sums)
 __global__ void __launch_bounds__(256)
-k_warm_residual(const float4 *__restrict__ x0s, const float4 *__restrict__ g,
-                const float4 *__restrict__ r, float4 *__restrict__ rw, LevelGeom G,
-                DD *part_a, DD *part_b)
+k_warm_aw(const float4 *__restrict__ w, const float4 *__restrict__ g,
+          const float4 *__restrict__ r, float4 *__restrict__ aw, LevelGeom G, DD *part_bw,
+          DD *part_waw, DD *part_bb)
 {
     const Dims4 &D = G.D;
     const int64_t n = (int64_t)D.st * D.T;
-    DD r2{0., 0.}, b2{0., 0.};
+    DD bw{0., 0.}, waw{0., 0.}, bb{0., 0.};
     FOR_XCD(c, n)
     {
         int i, j, k, t;
         decompose((uint32_t)c, G.C, i, j, k, t);
-        const float4 xc = x0s[c];
-        const float4 ax = interior(D, i, j, k, t) ? apply_A(x0s, c, xc, g[c], G) : apply_I(xc);
+        const float4 wc = w[c];
+        const float4 a = interior(D, i, j, k, t) ? apply_A(w, c, wc, g[c], G) : apply_I(wc);
         const float4 b = r[c];
-        const float4 rr = f4sub(b, ax);
-        rw[c] = rr;
-        dd_add(r2, (double)rr.x * rr.x);
-        dd_add(r2, (double)rr.y * rr.y);
-        dd_add(r2, (double)rr.z * rr.z);
-        dd_add(r2, (double)rr.w * rr.w);
-        dd_add(b2, (double)b.x * b.x);
-        dd_add(b2, (double)b.y * b.y);
-        dd_add(b2, (double)b.z * b.z);
-        dd_add(b2, (double)b.w * b.w);
+        aw[c] = a;
+        dd_add(bw, (double)b.x * (double)wc.x);
+        dd_add(bw, (double)b.y * (double)wc.y);
+        dd_add(bw, (double)b.z * (double)wc.z);
+        dd_add(bw, (double)b.w * (double)wc.w);
+        dd_add(waw, (double)wc.x * (double)a.x);
+        dd_add(waw, (double)wc.y * (double)a.y);
+        dd_add(waw, (double)wc.z * (double)a.z);
+        dd_add(waw, (double)wc.w * (double)a.w);
+        dd_add(bb, (double)b.x * (double)b.x);
+        dd_add(bb, (double)b.y * (double)b.y);
+        dd_add(bb, (double)b.z * (double)b.z);
+        dd_add(bb, (double)b.w * (double)b.w);
     }
-    r2 = block_dd(r2);
-    b2 = block_dd(b2);
+    bw = block_dd(bw);
+    waw = block_dd(waw);
+    bb = block_dd(bb);
     if (threadIdx.x == 0) {
-        part_a[blockIdx.x] = r2;
-        part_b[blockIdx.x] = b2;
+        part_bw[blockIdx.x] = bw;
+        part_waw[blockIdx.x] = waw;
+        part_bb[blockIdx.x] = bb;
     }
 }
 
-__global__ void k_warm_fin(CgState *st, const DD *pa, const DD *pb, int nb)
+// gamma = (b.w) / (w.A w), the energy-optimal multiple of the guess (0 if
+// w.A w is not positive and finite)
+__global__ void k_warm_gamma(CgState *st, const DD *pbw, const DD *pwaw, const DD *pbb, int nb)
 {
-    DD s, u;
+    DD s1, s2, s3;
+    float m;
+    fin_dd_max(pbw, nullptr, nb, s1, m);
+    fin_dd_max(pwaw, nullptr, nb, s2, m);
+    fin_dd_max(pbb, nullptr, nb, s3, m);
+    if (threadIdx.x == 0) {
+        const double wAw = s2.hi + s2.lo;
+        double gamma = 0.;
+        if (wAw > 0. && wAw < HUGE_VAL) gamma = (s1.hi + s1.lo) / wAw;
+        if (gamma != gamma) gamma = 0.;
+        st->gamma = gamma;
+        st->b2 = s3.hi + s3.lo;
+    }
+}
+
+// x0 = gamma w, r_w = b - gamma (A w) (in place), partial ||r_w||^2
+__global__ void __launch_bounds__(256)
+k_warm_scale(float4 *__restrict__ w, float4 *__restrict__ aw, const float4 *__restrict__ r,
+             int64_t n, const CgState *st, DD *part)
+{
+    const double gm = st->gamma;
+    DD r2{0., 0.};
+    FOR_XCD(c, n)
+    {
+        const float4 wc = w[c], a = aw[c], b = r[c];
+        float4 x, rr;
+        x.x = (float)(gm * (double)wc.x);
+        x.y = (float)(gm * (double)wc.y);
+        x.z = (float)(gm * (double)wc.z);
+        x.w = (float)(gm * (double)wc.w);
+        rr.x = (float)((double)b.x - gm * (double)a.x);
+        rr.y = (float)((double)b.y - gm * (double)a.y);
+        rr.z = (float)((double)b.z - gm * (double)a.z);
+        rr.w = (float)((double)b.w - gm * (double)a.w);
+        w[c] = x;
+        aw[c] = rr;
+        dd_add(r2, (double)rr.x * (double)rr.x);
+        dd_add(r2, (double)rr.y * (double)rr.y);
+        dd_add(r2, (double)rr.z * (double)rr.z);
+        dd_add(r2, (double)rr.w * (double)rr.w);
+    }
+    r2 = block_dd(r2);
+    if (threadIdx.x == 0) part[blockIdx.x] = r2;
+}
+
+__global__ void k_warm_fin(CgState *st, const DD *pa, int nb)
+{
+    DD s;
     float m;
     fin_dd_max(pa, nullptr, nb, s, m);
-    fin_dd_max(pb, nullptr, nb, u, m);
     if (threadIdx.x == 0) {
         st->r2 = s.hi + s.lo;
-        st->b2 = u.hi + u.lo;
         st->warm = (st->done == 0 && st->r2 <= st->b2) ? 1 : 0;
     }
 }
@@ -1126,12 +1182,13 @@ struct LevelWork {
         set_tm(ctx, D, ctx->tm_blocks_per_cu);
         cap = std::max(cap, nb_tm);
         st = Tmp(ctx, sizeof(CgState));
-        part = Tmp(ctx, (size_t)cap * (2 * sizeof(DD) + sizeof(float)) + 256);
+        part = Tmp(ctx, (size_t)cap * (3 * sizeof(DD) + sizeof(float)) + 256);
         FLOF_HIP(hipMemsetAsync(st.p, 0, sizeof(CgState), ctx->stream));
     }
     DD *pdd() const { return part.as<DD>(); }
     DD *pdd2() const { return part.as<DD>() + cap; }
-    float *pmax() const { return (float *)(part.as<DD>() + 2 * cap); }
+    DD *pdd3() const { return part.as<DD>() + 2 * cap; }
+    float *pmax() const { return (float *)(part.as<DD>() + 3 * cap); }
 };
 
 // one fused PCG iteration (three t-marching kernels and the two finalizers);
@@ -1279,12 +1336,21 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
                            warm_base, W.q.as<float4>(), D, G.C, mDx);
         check_launch("k_warm_guess");
         Tmp rw(ctx, (size_t)n * 16);
-        hipLaunchKernelGGL(k_warm_residual, dim3(nb), dim3(256), 0, ctx->stream,
-                           W.q.as<float4>(), W.g.as<float4>(), W.r.as<float4>(),
-                           rw.as<float4>(), G, W.pdd(), W.pdd2());
-        check_launch("k_warm_residual");
-        hipLaunchKernelGGL(k_warm_fin, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
-                           W.pdd2(), nb);
+        hipLaunchKernelGGL(k_warm_aw, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const float4 *)W.q.as<float4>(), (const float4 *)W.g.as<float4>(),
+                           (const float4 *)W.r.as<float4>(), rw.as<float4>(), G, W.pdd(),
+                           W.pdd2(), W.pdd3());
+        check_launch("k_warm_aw");
+        hipLaunchKernelGGL(k_warm_gamma, dim3(1), dim3(256), 0, ctx->stream, st,
+                           (const DD *)W.pdd(), (const DD *)W.pdd2(), (const DD *)W.pdd3(),
+                           nb);
+        check_launch("k_warm_gamma");
+        hipLaunchKernelGGL(k_warm_scale, dim3(nb), dim3(256), 0, ctx->stream,
+                           W.q.as<float4>(), rw.as<float4>(), (const float4 *)W.r.as<float4>(),
+                           n, (const CgState *)st, W.pdd());
+        check_launch("k_warm_scale");
+        hipLaunchKernelGGL(k_warm_fin, dim3(1), dim3(256), 0, ctx->stream, st,
+                           (const DD *)W.pdd(), nb);
         check_launch("k_warm_fin");
         hipLaunchKernelGGL(k_warm_adopt, dim3(nb), dim3(256), 0, ctx->stream,
                            W.q.as<float4>(), rw.as<float4>(), W.prec.as<float4>(),

@@ -497,20 +497,35 @@ static CgResult pcg(const std::vector<R> &offd, const std::vector<R> &blockd,
     const double acc = accuracy * residual;
     const double resIni = residual;
     if (x0) {
-        // Warm start (extension): x = x0 and r = b - A x0, kept only when
-        // ||b - A x0||_2 <= ||b||_2, else the cold start x = 0 is used.  The
-        // stopping threshold stays relative to the cold-start residual.
-        applyMat(offd, blockd, off, *x0, tmp);
-        double r2 = 0., b2 = 0.;
+        // Warm start (extension): the guess w = x0 is scaled by the
+        // energy-optimal factor gamma = (b.w) / (w.A w) (the multiple of w
+        // closest to the solution in the A-norm, 0 if w.A w is not positive),
+        // then x = gamma w and r = b - gamma (A w), kept only when
+        // ||r||_2 <= ||b||_2, else the cold start x = 0 is used.  The stopping
+        // threshold stays relative to the cold-start residual.
+        const std::vector<R> &w = *x0;
+        applyMat(offd, blockd, off, w, tmp);
+        DSum bw, waw, bb;
         for (size_t i = 0; i < n; ++i) {
-            const R ri = rhs[i] - tmp[i];
-            r2 += (double)ri * ri;
-            b2 += (double)rhs[i] * rhs[i];
+            bw.add((double)rhs[i] * (double)w[i]);
+            waw.add((double)w[i] * (double)tmp[i]);
+            bb.add((double)rhs[i] * (double)rhs[i]);
         }
-        if (r2 <= b2) {
+        const double wAw = waw.value();
+        double gamma = 0.;
+        if (wAw > 0. && wAw < HUGE_VAL) gamma = bw.value() / wAw;
+        if (gamma != gamma) gamma = 0.;
+        std::vector<R> xw(n), rw(n);
+        DSum r2;
+        for (size_t i = 0; i < n; ++i) {
+            xw[i] = (R)(gamma * (double)w[i]);
+            rw[i] = (R)((double)rhs[i] - gamma * (double)tmp[i]);
+            r2.add((double)rw[i] * (double)rw[i]);
+        }
+        if (r2.value() <= bb.value()) {
             out.warm = 1;
-            x = *x0;
-            for (size_t i = 0; i < n; ++i) res[i] = rhs[i] - tmp[i];
+            x = xw;
+            res = rw;
             residual = maxSigned(res);
             if (residual <= acc) {
                 out.iters = 0;

@@ -274,3 +274,33 @@ def test_projection_planar_level_sets():
     assert np.abs(core[..., 1:]).max() == 0.0
     assert np.abs(core[..., 0] - (a - b)).max() <= 0.125
     np.testing.assert_allclose(v, -dst, atol=0)
+
+
+def test_warm_start_energy_optimal_scaling():
+    """The warm start is scaled by gamma = b.w / w.A w: a multiple of the
+    converged flow is rescaled to it (the solve needs no iterations), a flow
+    orthogonal in the energy sense is dropped (gamma 0), the zero guess is
+    the cold start."""
+    N, T = 8, 8
+    i0, i1 = drop_pair(N, T)
+    w, e = 0.1, 1e-4
+    p = default_params(wSmooth=w, wEnergy=e, cgAccuracy=1e-9, cgMaxIter=5000)
+    zero = np.zeros(i0.shape + (4,))
+    v, it, _, _ = O.optical_flow4d(zero, i0, i1, p)
+    p2 = default_params(wSmooth=w, wEnergy=e, cgAccuracy=1e-6, cgMaxIter=5000)
+    for s in (1.0, 2.5, -0.7):
+        vw, it_w, _, st = O.optical_flow4d(zero, i0, i1, p2, warm=s * v)
+        assert st == 0 and it_w <= 1, (s, it_w)
+        np.testing.assert_allclose(vw, v, atol=1e-5 * np.abs(v).max())
+    # the exact Galerkin factor: gamma = b.w / w.A w on the assembled system
+    A, b, ints = assemble(i0, i1, w, e)
+    g = np.random.default_rng(0).standard_normal(v.shape)
+    wv = np.concatenate([g[c] for c in ints]) / N
+    gamma = (b @ wv) / (wv @ (A @ wv))
+    p1 = default_params(wSmooth=w, wEnergy=e, cgAccuracy=1e-6, cgMaxIter=0)
+    vg, it_g, _, st_g = O.optical_flow4d(zero, i0, i1, p1, warm=g)
+    # with no iterations the result is the adopted guess gamma w (this guess
+    # passes the ||b - gamma A w|| <= ||b|| test)
+    assert np.linalg.norm(b - gamma * (A @ wv)) <= np.linalg.norm(b)
+    ref = scatter(gamma * wv, ints, i0.shape) * N
+    np.testing.assert_allclose(vg, ref, atol=1e-12 * np.abs(ref).max())
