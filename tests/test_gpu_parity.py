@@ -296,9 +296,10 @@ def test_multiscale_flof_params_close():
 
 
 def test_warm_start_bit_exact_and_fallback():
-    """Warm start (extension): the guess (warm - incoming flow) seeds every
-    level's CG; kept only when ||b - A x0|| <= ||b||.  Bit-exact against the
-    oracle with the same guess; a useless guess falls back to the cold start
+    """Warm start (extension): the guess w = (warm - incoming flow) of every
+    level is scaled by gamma = b.w / w.A w and kept only when
+    ||b - gamma A w|| <= ||b||.  Bit-exact against the oracle with the same
+    guess; a guess that fails the test (NaN) falls back to the cold start
     (identical to the plain solve)."""
     N, T, mg = 16, 16, 4
     i0, i1 = _drops(N, T)
@@ -326,10 +327,18 @@ def test_warm_start_bit_exact_and_fallback():
     assert [l["warm_used"] for l in info["levels"]] == [l["warm_used"] for l in r["levels"]]
     assert np.array_equal(vel.to_numpy(), ref)
     assert info["levels"][0]["iters"][0] < cold_info["levels"][0]["iters"][0]
-    # a wild guess is rejected: same as the cold solve
-    bad = vec4(s, (RNG.standard_normal(vcold.shape) * 1e3).astype(np.float32))
+    # a wild guess is scaled down to almost nothing: still bit-exact
+    wild = (RNG.standard_normal(vcold.shape) * 1e3).astype(np.float32)
     vb = s.create(M.Grid4Vec4)
-    M.opticalFlowMultiscale4d(vel=vb, i0=g0, i1=g1, warm=bad, **prm)
-    binfo = M.getLastSolveInfo().summary()
-    assert [l["warm_used"] for l in binfo["levels"]] == [0] * len(binfo["levels"])
-    assert np.array_equal(vb.to_numpy(), vcold)
+    M.opticalFlowMultiscale4d(vel=vb, i0=g0, i1=g1, warm=vec4(s, wild), **prm)
+    with O.sum_mode(1):
+        refw, _, _ = O.multiscale4d(np.zeros_like(vcold), i0, i1, default_params(**prm),
+                                    warm=wild)
+    assert np.array_equal(vb.to_numpy(), refw)
+    # a NaN guess fails the residual test on every level: the cold solve
+    nan = np.full(vcold.shape, np.nan, np.float32)
+    vn = s.create(M.Grid4Vec4)
+    M.opticalFlowMultiscale4d(vel=vn, i0=g0, i1=g1, warm=vec4(s, nan), **prm)
+    ninfo = M.getLastSolveInfo().summary()
+    assert [l["warm_used"] for l in ninfo["levels"]] == [0] * len(ninfo["levels"])
+    assert np.array_equal(vn.to_numpy(), vcold)
