@@ -52,6 +52,10 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="enqueue CG iterations as plain launches instead of "
                          "captured graphs (for profilers that trace graphs poorly)")
+    ap.add_argument("--batch-workload", action="store_true",
+                    help="16 solves (forward and backward of 8 seeded drop pairs, "
+                         "N in {48,64,80}, T in {32,48,64}) one after another and "
+                         "as one batch on worker streams; prints its own JSON line")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="short workload for rocprofv3 --pmc runs: a 256 MiB "
                          "grid copy (calibration) and 3 level-0 PCG iterations")
@@ -70,6 +74,9 @@ def main():
                                  FLOF_KT_WARP, FLOF_PSTORE_BF16)
 
     L = _lib.lib()
+    if args.batch_workload:
+        batch_workload(M, L)
+        return
     s = M.Solver(gridSize=M.vec3(N, N, N), fourthDim=N)
     ctx = M.context()
     i0 = s.create(M.Grid4Real)
@@ -232,6 +239,59 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.cpu_iters, iters, cells, ms_step)
         out["speedup_vs_cpu"] = out["cpu_baseline"]["s_per_solve"] / (ms_step / 1000.0)
     print(json.dumps(out), flush=True)
+
+
+def batch_workload(M, L):
+    """Forward and backward deformations of 8 seeded drop configurations of
+    uneven sizes: sequential solves vs one batch on worker streams (same
+    results, checked), with the launch-shape autotuner on."""
+    rng = np.random.default_rng(4)
+    jobs = []
+    for k in range(8):
+        n = int(rng.choice([48, 64, 80]))
+        t = int(rng.choice([32, 48, 64]))
+        sv = M.Solver(gridSize=M.vec3(n, n, n), fourthDim=t)
+        a, b = sv.create(M.Grid4Real), sv.create(M.Grid4Real)
+        M.dropSdf4d(a, seed=100 + 2 * k, nspheres=1 + k % 3, rmin=0.08, rmax=0.18)
+        M.dropSdf4d(b, seed=101 + 2 * k, nspheres=1 + k % 3, rmin=0.08, rmax=0.18)
+        for (x, y) in ((a, b), (b, a)):
+            jobs.append((sv, x, y, sv.create(M.Grid4Vec4), (n, t)))
+    prm = dict(PARAMS, cgMaxIter=FIXED_RESIDUAL_CAP)
+    L.flof_ctx_set_autotune(M.context(), 1)
+    log("bench: batch warm-up")
+    for sv, x, y, v, _ in jobs[:2]:
+        M.opticalFlowMultiscale4d(vel=v, i0=x, i1=y, **prm)
+    M.sync()
+    t0 = time.perf_counter()
+    seq = []
+    for sv, x, y, v, _ in jobs:
+        v.clear()
+        M.opticalFlowMultiscale4d(vel=v, i0=x, i1=y, **prm)
+        seq.append(v.to_numpy())
+    M.sync()
+    t_seq = time.perf_counter() - t0
+    log("bench: sequential %.2f s" % t_seq)
+    res = {}
+    for conc in (2, 4):
+        for _, _, _, v, _ in jobs:
+            v.clear()
+        M.sync()
+        t0 = time.perf_counter()
+        errs, infos = M.opticalFlowMultiscale4dBatch([j[3] for j in jobs], [j[1] for j in jobs],
+                                                     [j[2] for j in jobs], concurrency=conc,
+                                                     **prm)
+        M.sync()
+        tb = time.perf_counter() - t0
+        same = all(np.array_equal(j[3].to_numpy(), seq[q]) for q, j in enumerate(jobs))
+        res["concurrency_%d" % conc] = {"s": tb, "solves_per_s": len(jobs) / tb,
+                                        "identical_to_sequential": same}
+        log("bench: batch x%d %.2f s identical %s" % (conc, tb, same))
+    cells = sum(j[4][0] ** 3 * j[4][1] for j in jobs)
+    print(json.dumps({"workload": "16 solves: forward+backward of 8 seeded drop pairs, "
+                                  "N in {48,64,80}, T in {32,48,64}, P_plain, residual 1e-5",
+                      "sizes": [j[4] for j in jobs], "total_cells": cells,
+                      "sequential_s": t_seq, "sequential_solves_per_s": len(jobs) / t_seq,
+                      **res}), flush=True)
 
 
 def committed_traffic(kernel):
