@@ -198,6 +198,22 @@ def main():
         "ms_per_step_without_kernel_events": 1000.0 * min(t_plain),
     }
 
+    # host-buffer boundary: the inputs come from and the flow goes to host
+    # memory (pageable numpy arrays): copies around one solve
+    h0, h1 = i0.to_numpy(), i1.to_numpy()
+    M.sync()
+    t0 = time.perf_counter()
+    i0.from_numpy(h0)
+    i1.from_numpy(h1)
+    M.sync()
+    t_up = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hv = vel.to_numpy()
+    t_down = time.perf_counter() - t0
+    out["pcie_inclusive"] = {"upload_ms": 1000.0 * t_up, "download_ms": 1000.0 * t_down,
+                             "ms_per_step": ms_step + 1000.0 * (t_up + t_down),
+                             "bytes": h0.nbytes + h1.nbytes + hv.nbytes}
+
     # the same solve with the reference's iteration cap (1000)
     t0 = time.perf_counter()
     solve(1000)
