@@ -342,3 +342,30 @@ def test_warm_start_bit_exact_and_fallback():
     ninfo = M.getLastSolveInfo().summary()
     assert [l["warm_used"] for l in ninfo["levels"]] == [0] * len(ninfo["levels"])
     assert np.array_equal(vn.to_numpy(), vcold)
+
+
+def test_multiscale_ragged_sizes_bit_exact():
+    """Uneven X, Y, Z, T (odd halvings in the pyramid) and the early exits:
+    identical inputs give a zero right-hand side (status 3, no iterations)."""
+    X, Y, Z, T = 21, 18, 16, 13
+    a = O.drop_sdf(X, Y, Z, T, seed=41)
+    b = O.drop_sdf(X, Y, Z, T, seed=42)
+    s = solver(X, Y, Z, T)
+    prm = dict(O.P_PLAIN, minGridSize=6)
+    vel = s.create(M.Grid4Vec4)
+    err = M.opticalFlowMultiscale4d(vel=vel, i0=real(s, a), i1=real(s, b), **prm)
+    info = M.getLastSolveInfo().summary()
+    with O.sum_mode(1):
+        ref, rerr, rinfo = O.multiscale4d(np.zeros((T, Z, Y, X, 4), np.float32), a, b,
+                                          default_params(**prm))
+    say("ragged dims", [l["dims"] for l in info["levels"]], "iters",
+        [l["iters"] for l in info["levels"]])
+    assert [l["dims"] for l in info["levels"]] == [l["dims"] for l in rinfo.summary()["levels"]]
+    assert [l["iters"] for l in info["levels"]] == [l["iters"] for l in rinfo.summary()["levels"]]
+    assert np.array_equal(vel.to_numpy(), ref)
+    assert err == np.float32(rerr)
+    v2 = s.create(M.Grid4Vec4)
+    M.opticalFlow4d(v2, real(s, a), real(s, a), wSmooth=0.1, wEnergy=1e-4)
+    lv = M.getLastSolveInfo().summary()["levels"][0]
+    assert lv["status"] == [3] and lv["iters"] == [0]
+    assert not v2.to_numpy().any()
