@@ -40,6 +40,12 @@
 // non-temporal access to data a kernel touches once (1: r and x of the
 // update kernel; 2: also p's own load and store in the direction kernel).
 // Measured at 64^4: 626 -> 565 us per PCG iteration.
+// 1: t-march work items dealt XCD-contiguously (blockIdx % 8 = XCD);
+// 0: in launch order.  Launch order measured 2.5% faster per PCG iteration
+// at 64^4 (its z-neighbour chunks, 16 blocks apart, share an XCD).
+#ifndef FLOF_TM_XCD_MAP
+#define FLOF_TM_XCD_MAP 0
+#endif
 #ifndef FLOF_STREAM_NT
 #define FLOF_STREAM_NT 2
 #endif
@@ -464,10 +470,10 @@ k_cg_apply(const float4 *__restrict__ p, const float4 *__restrict__ g,
 
 // t-marching work items: (3D cell c3, t segment); the thread keeps the
 // t-1, t, t+1 values of its stencil operands in registers, so every value is
-// fetched once from HBM; the x/y/z neighbours at time t are fetched through L2, which holds
-// the few t-planes the co-resident workgroups are working on.  Workgroups of
-// one XCD (blockIdx % 8) take a contiguous range of work items, so the
-// neighbouring rows and planes of a workgroup live in the same L2.
+// fetched once from HBM; the x/y/z neighbours at time t are fetched through
+// the caches, which hold the few t-planes the co-resident workgroups are
+// working on (every workgroup of a launch is resident at 64^4).  Work items
+// follow the launch order (FLOF_TM_XCD_MAP selects the XCD-contiguous order).
 struct TMarch {
     int64_t n3;      // X*Y*Z
     int nblk3;       // workgroups per t segment (ceil(n3 / 256))
@@ -618,7 +624,12 @@ __device__ __forceinline__ TmCell tm_cell(const Dims4 &D, const TMarch &M)
 {
     TmCell w;
     const int per = gridDim.x / FLOF_NXCD;
+#if FLOF_TM_XCD_MAP
     const int L = (blockIdx.x % FLOF_NXCD) * per + blockIdx.x / FLOF_NXCD;
+#else
+    const int L = blockIdx.x;
+    (void)per;
+#endif
     w.live = false;
     w.c3 = 0;
     w.t0 = w.t1 = 0;
