@@ -637,9 +637,9 @@ __device__ __forceinline__ TmCell tm_cell(const Dims4 &D, const TMarch &M)
     if (L >= M.total) return w;
     const int seg = L / M.nblk3;
     const int b3 = L - seg * M.nblk3;
-    w.c3 = (int64_t)b3 * blockDim.x + threadIdx.x;
     w.t0 = seg * M.seglen;
     w.t1 = min(D.T, w.t0 + M.seglen);
+    w.c3 = (int64_t)b3 * blockDim.x + threadIdx.x;
     if (w.c3 >= M.n3 || w.t0 >= w.t1) return w;
     const int i = (int)(w.c3 % D.X);
     const int j = (int)((w.c3 / D.X) % D.Y);
