@@ -290,6 +290,23 @@ int flof_advect_time_slice(flof_grid4d *defo, float *dst_dev, int dx, int dy,
                            const float defoFactor[4],
                            const float overrideSize[4], float overrideTimeOff,
                            int zeroVel, float defoAniFac);
+/* loadAdvectTimeSlice_OptRun with deformation volumes (useDefoVols;
+ * optflow4d.cpp:1786-1870, 2013-2085): 2 or 3 VEC4 volumes of one size,
+ * each seen through a window of int(T * max(0.2, partialLoadFac)) slices
+ * around the time; 2 volumes: vt = alpha dv1(p - dv2(p)) + thirdAlpha dv2(p)
+ * (doAligned: the two-pass aligned blend), 3 volumes: the chained blend with
+ * fourthAlpha; then the lookup with alpha 1 over the bnd-10 interior. */
+int flof_advect_time_slice_vols(flof_grid4d *const *vols, int nvols,
+                                float *dst_dev, int dx, int dy, int dz,
+                                flof_grid4d *phi, float time, float blendAlpha,
+                                float loadTimeScale, const float defoOffset[4],
+                                const float defoScale[4],
+                                const float defoFactor[4],
+                                const float overrideSize[4],
+                                float overrideTimeOff, int zeroVel, int bordSkip,
+                                float defoAniFac, int doAligned,
+                                float partialLoadFac, float thirdAlpha,
+                                float fourthAlpha);
 /* Batched form: warps nframes output frames (times time0 + f*dtime) in one
  * launch; frame f is written at dst_dev + f*dx*dy*dz. */
 int flof_advect_time_slices_opt(flof_grid4d *defo, float *dst_dev, int dx,
@@ -334,18 +351,24 @@ int flof_load_place_grid4d_uni(const char *fname, flof_grid4d *phi,
                                float overrideTimeOff, int overrideGoodRegion,
                                float loadTimeScale, int rescaleSdfValues,
                                float sdfIsoOff, float repeatStartFrame);
-/* loadAdvectTimeSlice_OptInit (one deformation volume, useDefoVols off):
- * loads the M4T2 deformation into device memory; the handle replaces the
- * reference's int-keyed global map. */
-int flof_warp_state_open(flof_ctx *ctx, const char *fname,
+/* loadAdvectTimeSlice_OptInit(ID, fname1, useDefoVols, doAligned,
+ * partialLoadFac) (optflow4d.cpp:1871-1913): loads the M4T2 deformation into
+ * device memory; the handle replaces the reference's int-keyed global map. */
+int flof_warp_state_open(flof_ctx *ctx, const char *fname, int useDefoVols,
+                         int doAligned, float partialLoadFac,
                          flof_warp_state **out);
-/* loadAdvectTimeSlice_OptRun with the state's deformation. */
+/* loadAdvectTimeSlice_OptAdd (optflow4d.cpp:1914-1928): another deformation
+ * volume (ignored without useDefoVols, as in the reference). */
+int flof_warp_state_add(flof_warp_state *s, const char *fname);
+/* loadAdvectTimeSlice_OptRun (optflow4d.cpp:1951-2105) with the state's
+ * deformation(s); arguments in the reference's order. */
 int flof_warp_state_run(flof_warp_state *s, float *dst_dev, int dx, int dy,
                         int dz, flof_grid4d *phi, float time, float blendAlpha,
                         float loadTimeScale, const float defoOffset[4],
                         const float defoScale[4], const float defoFactor[4],
                         const float overrideSize[4], float overrideTimeOff,
-                        int zeroVel, int bordSkip, float defoAniFac);
+                        int zeroVel, float thirdAlpha, int bordSkip,
+                        float fourthAlpha, float defoAniFac);
 /* loadAdvectTimeSlice_Finish. */
 int flof_warp_state_close(flof_warp_state *s);
 

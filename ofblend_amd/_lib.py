@@ -92,6 +92,9 @@ SIGNATURES = {
                                     _FP, _FP, _FP, _F, _I, _I, _F], _I),
     "flof_advect_time_slice": ([_P, _P, _I, _I, _I, _P, _F, _F, _F, _FP, _FP, _FP,
                                 _FP, _F, _I, _F], _I),
+    "flof_advect_time_slice_vols": ([C.POINTER(_P), _I, _P, _I, _I, _I, _P, _F, _F, _F,
+                                     _FP, _FP, _FP, _FP, _F, _I, _I, _F, _I, _F, _F, _F],
+                                    _I),
     "flof_advect_time_slices_opt": ([_P, _P, _I, _I, _I, _I, _F, _F, _P, _F,
                                      _F, _FP, _FP, _FP, _I, _F], _I),
     "flof_drop_sdf4d": ([_P, _U64, _I, _D, _D, _D], _I),
@@ -107,9 +110,10 @@ SIGNATURES = {
     "flof_grid4d_load": ([_P, C.c_char_p], _I),
     "flof_load_place_grid4d_uni": ([C.c_char_p, _P, _FP, _FP, _I, _I, _I, _F, _FP,
                                     _F, _I, _F, _I, _F, _F], _I),
-    "flof_warp_state_open": ([_P, C.c_char_p, C.POINTER(_P)], _I),
+    "flof_warp_state_open": ([_P, C.c_char_p, _I, _I, _F, C.POINTER(_P)], _I),
+    "flof_warp_state_add": ([_P, C.c_char_p], _I),
     "flof_warp_state_run": ([_P, _P, _I, _I, _I, _P, _F, _F, _F, _FP, _FP, _FP, _FP,
-                             _F, _I, _I, _F], _I),
+                             _F, _I, _F, _I, _F, _F], _I),
     "flof_warp_state_close": ([_P], _I),
     "flof_warmnet_num_params": ([], _I),
     "flof_warmnet_create": ([_P, _U64, C.POINTER(_P)], _I),

@@ -100,8 +100,11 @@ struct flof_ctx {
 };
 
 struct flof_warp_state {
-    flof_ctx *ctx;
-    flof_grid4d *defo;  // the deformation, device resident
+    flof_ctx *ctx = nullptr;
+    std::vector<flof_grid4d *> defos;  // deformation volumes, device resident
+    bool use_vols = false;             // useDefoVols of loadAdvectTimeSlice_OptInit
+    bool aligned = false;              // doAligned
+    float partial = 0.2f;              // partialLoadFac
 };
 
 struct flof_grid4d {

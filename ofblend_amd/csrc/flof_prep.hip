@@ -435,11 +435,34 @@ int flof_load_place_grid4d_uni(const char *fname, flof_grid4d *phi, const float 
     API_CATCH(ctx)
 }
 
-int flof_warp_state_open(flof_ctx *ctx, const char *fname, flof_warp_state **out)
+int flof_warp_state_open(flof_ctx *ctx, const char *fname, int useDefoVols, int doAligned,
+                         float partialLoadFac, flof_warp_state **out)
 {
     API_TRY
     FLOF_REQUIRE(ctx && fname && out, FLOF_ERR_ARG, "loadAdvectTimeSlice_OptInit: null argument");
     *out = nullptr;
+    flof_warp_state *s = new flof_warp_state();
+    s->ctx = ctx;
+    s->use_vols = useDefoVols != 0;
+    s->aligned = doAligned != 0;
+    s->partial = partialLoadFac;
+    const int rc = flof_warp_state_add(s, fname);
+    if (rc != FLOF_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    API_CATCH(ctx)
+}
+
+int flof_warp_state_add(flof_warp_state *s, const char *fname)
+{
+    flof_ctx *ctx = s ? s->ctx : nullptr;
+    API_TRY
+    FLOF_REQUIRE(s && fname, FLOF_ERR_ARG, "loadAdvectTimeSlice_OptAdd: null argument");
+    // without deformation volumes the reference ignores the call
+    if (!s->defos.empty() && !s->use_vols) return FLOF_OK;
+    FLOF_REQUIRE((int)s->defos.size() < 9, FLOF_ERR_RANGE, "Too many defovolumes loaded!");
     int d[4];
     uni_size(fname, d);
     FLOF_REQUIRE(d[0] >= 2 && d[1] >= 2 && d[2] >= 2 && d[3] >= 1, FLOF_ERR_DIM,
@@ -452,7 +475,7 @@ int flof_warp_state_open(flof_ctx *ctx, const char *fname, flof_warp_state **out
         flof_grid4d_destroy(g);
         return rc;
     }
-    *out = new flof_warp_state{ctx, g};
+    s->defos.push_back(g);
     API_CATCH(ctx)
 }
 
@@ -460,19 +483,26 @@ int flof_warp_state_run(flof_warp_state *s, float *dst, int dx, int dy, int dz,
                         flof_grid4d *phi, float time, float blendAlpha, float loadTimeScale,
                         const float defoOffset[4], const float defoScale[4],
                         const float defoFactor[4], const float overrideSize[4],
-                        float overrideTimeOff, int zeroVel, int bordSkip, float defoAniFac)
+                        float overrideTimeOff, int zeroVel, float thirdAlpha, int bordSkip,
+                        float fourthAlpha, float defoAniFac)
 {
-    if (!s) return FLOF_ERR_ARG;
-    return flof_advect_time_slice_opt(s->defo, dst, dx, dy, dz, phi, time, blendAlpha,
-                                      loadTimeScale, defoOffset, defoScale, defoFactor,
-                                      overrideSize, overrideTimeOff, zeroVel, bordSkip,
-                                      defoAniFac);
+    if (!s || s->defos.empty()) return FLOF_ERR_ARG;
+    if (!s->use_vols)
+        return flof_advect_time_slice_opt(s->defos[0], dst, dx, dy, dz, phi, time, blendAlpha,
+                                          loadTimeScale, defoOffset, defoScale, defoFactor,
+                                          overrideSize, overrideTimeOff, zeroVel, bordSkip,
+                                          defoAniFac);
+    return flof_advect_time_slice_vols(s->defos.data(), (int)s->defos.size(), dst, dx, dy, dz,
+                                       phi, time, blendAlpha, loadTimeScale, defoOffset,
+                                       defoScale, defoFactor, overrideSize, overrideTimeOff,
+                                       zeroVel, bordSkip, defoAniFac, s->aligned ? 1 : 0,
+                                       s->partial, thirdAlpha, fourthAlpha);
 }
 
 int flof_warp_state_close(flof_warp_state *s)
 {
     if (!s) return FLOF_ERR_ARG;
-    flof_grid4d_destroy(s->defo);
+    for (flof_grid4d *g : s->defos) flof_grid4d_destroy(g);
     delete s;
     return FLOF_OK;
 }

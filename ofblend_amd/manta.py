@@ -540,16 +540,21 @@ _LATS = {}
 
 def loadAdvectTimeSlice_OptInit(ID, fname1, useDefoVols=False, doAligned=False,
                                 partialLoadFac=0.2):
-    """optflow4d.cpp:1871-1913 for one deformation volume (useDefoVols off);
-    the deformation is loaded to the GPU once."""
-    if useDefoVols:
-        raise FlofError(-1, "deformation volume blending is not available; "
-                            "use useDefoVols=False")
+    """optflow4d.cpp:1871-1913: the deformation is loaded to the GPU once."""
     loadAdvectTimeSlice_Finish(ID)
     h = C.c_void_p()
     ctx = context()
-    check(_lib.lib().flof_warp_state_open(ctx, str(fname1).encode(), C.byref(h)), ctx)
+    check(_lib.lib().flof_warp_state_open(ctx, str(fname1).encode(), int(useDefoVols),
+                                          int(doAligned), partialLoadFac, C.byref(h)), ctx)
     _LATS[ID] = h
+
+
+def loadAdvectTimeSlice_OptAdd(ID, fname):
+    """optflow4d.cpp:1914-1928: another deformation volume."""
+    if ID not in _LATS:
+        return
+    ctx = context()
+    check(_lib.lib().flof_warp_state_add(_LATS[ID], str(fname).encode()), ctx)
 
 
 def loadAdvectTimeSlice_OptRun(ID, fname, dst, phi, time, blendAlpha, loadTimeScale,
@@ -564,8 +569,25 @@ def loadAdvectTimeSlice_OptRun(ID, fname, dst, phi, time, blendAlpha, loadTimeSc
     check(_lib.lib().flof_warp_state_run(
         _LATS[ID], dst.ptr, dx, dy, dz, phi.h, time, blendAlpha, loadTimeScale,
         floats(*defoOffset), floats(*defoScale), floats(*defoFactor),
-        floats(*overrideSize), overrideTimeOff, int(zeroVel), bordSkip, defoAniFac),
-        phi.ctx)
+        floats(*overrideSize), overrideTimeOff, int(zeroVel), thirdAlpha, bordSkip,
+        fourthAlpha, defoAniFac), phi.ctx)
+
+
+def loadAdvectTimeSliceFromDefoVols(vols, dst, phi, time, blendAlpha, loadTimeScale=1.,
+                                    defoOffset=Vec4(0.), defoScale=Vec4(1.),
+                                    defoFactor=Vec4(1.), overrideSize=Vec4(-1.),
+                                    overrideTimeOff=0., zeroVel=False, thirdAlpha=0.,
+                                    bordSkip=1, fourthAlpha=0., defoAniFac=1.,
+                                    doAligned=False, partialLoadFac=0.2):
+    """loadAdvectTimeSlice_OptRun with deformation volumes already on the GPU
+    (a list of 2 or 3 Grid4Vec4, optflow4d.cpp:2013-2085)."""
+    dz, dy, dx = dst.shape
+    arr = (C.c_void_p * len(vols))(*[v.h.value for v in vols])
+    check(_lib.lib().flof_advect_time_slice_vols(
+        arr, len(vols), dst.ptr, dx, dy, dz, phi.h, time, blendAlpha, loadTimeScale,
+        floats(*defoOffset), floats(*defoScale), floats(*defoFactor),
+        floats(*overrideSize), overrideTimeOff, int(zeroVel), bordSkip, defoAniFac,
+        int(doAligned), partialLoadFac, thirdAlpha, fourthAlpha), phi.ctx)
 
 
 def loadAdvectTimeSlice(dummyID, fname, dst, phi, time, blendAlpha, loadTimeScale,

@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <stdexcept>
 #include <vector>
 
 #include "../include/flof.h"
@@ -1081,6 +1082,35 @@ void placeFrames(R *phi, const Dims &D, const R *frames, int sx, int sy, int sz,
     }
 }
 
+// knSemiLagrangeLookupSlice4d(_WithVel), optflow4d.cpp:1622-1665: vt is the
+// velocity slice at deformation resolution; bnd 10 (with the border skip) for
+// the cached variant, 1 for the other
+template <class R>
+void lookupSlice(const V4<R> *vt, const Dims &Dv, R *dst, int dx, int dy, int dz,
+                 const R *phi, const Dims &Dp, R tl, R alpha, const V4<R> &sf,
+                 const V4<R> &off2, const V4<R> &fac, int bordSkip, bool optimized)
+{
+    const int bnd = optimized ? 10 : 1;
+#pragma omp parallel for schedule(static)
+    for (int k = bnd; k < dz - bnd; ++k)
+        for (int j = bnd; j < dy - bnd; ++j)
+            for (int i = bnd; i < dx - bnd; ++i) {
+                if (optimized && !(i >= bordSkip && j >= bordSkip && k >= bordSkip &&
+                                   i < dx - bordSkip && j < dy - bordSkip &&
+                                   k < dz - bordSkip))
+                    continue;
+                const R px = (R)i * sf.x + off2.x, py = (R)j * sf.y + off2.y,
+                        pz = (R)k * sf.z + off2.z;
+                const V4<R> v = interp3<R, V4<R>>(vt, Dv.X, Dv.Y, Dv.Z, Dv.sz(), px, py, pz) *
+                                fac;
+                const V4<R> pos =
+                    V4<R>((R)(i + 0.5f), (R)(j + 0.5f), (R)(k + 0.5f), (R)(tl + 0.5f)) -
+                    v * alpha;
+                dst[i + (idx_t)dx * j + (idx_t)dx * dy * k] =
+                    interp4<R, R>(phi, Dp, pos.x, pos.y, pos.z, pos.t);
+            }
+}
+
 // frame warp, optflow4d.cpp:1951-2105 (one deformation volume, slices taken
 // from the in-memory 4D defo), and its non-cached twin :1671-1760
 template <class R>
@@ -1121,29 +1151,103 @@ void warpSlice(const V4<R> *defo, const Dims &Dv, R *dst, int dx, int dy, int dz
         }
         vt[c] = zeroVel ? V4<R>(0) : v;
     }
-    const V4<R> fac = defoFactor * V4<R>(defoAniFac);
-    const R tl = time + overrideTimeOff;
-    const int bnd = optimized ? 10 : 1;
-#pragma omp parallel for schedule(static)
-    for (int k = bnd; k < dz - bnd; ++k)
-        for (int j = bnd; j < dy - bnd; ++j)
-            for (int i = bnd; i < dx - bnd; ++i) {
-                if (optimized && !(i >= bordSkip && j >= bordSkip && k >= bordSkip &&
-                                   i < dx - bordSkip && j < dy - bordSkip &&
-                                   k < dz - bordSkip))
-                    continue;
-                const R px = (R)i * sf.x + off2.x, py = (R)j * sf.y + off2.y,
-                        pz = (R)k * sf.z + off2.z;
-                const V4<R> v = interp3<R, V4<R>>(vt.data(), Dv.X, Dv.Y, Dv.Z,
-                                                  Dv.sz(), px, py, pz) *
-                                fac;
-                const V4<R> pos =
-                    V4<R>((R)(i + 0.5f), (R)(j + 0.5f), (R)(k + 0.5f),
-                          (R)(tl + 0.5f)) -
-                    v * blendAlpha;
-                dst[i + (idx_t)dx * j + (idx_t)dx * dy * k] =
-                    interp4<R, R>(phi, Dp, pos.x, pos.y, pos.z, pos.t);
-            }
+    lookupSlice<R>(vt.data(), Dv, dst, dx, dy, dz, phi, Dp, time + overrideTimeOff,
+                   blendAlpha, sf, off2, defoFactor * V4<R>(defoAniFac), bordSkip,
+                   optimized);
+}
+
+// Deformation volumes of loadAdvectTimeSlice_OptInit(useDefoVols=true) /
+// _OptAdd / _OptRun (optflow4d.cpp:1786-1870, 1871-1929, 2013-2085): each
+// volume is seen through a window of W = int(dimT * max(0.2, partialLoadFac))
+// slices, slice tl holding file slice clamp(t - W/2 + tl, 0, dimT - 1); the
+// velocity slice vt is combined from the volumes at window time
+// srcTime - (t - W/2), then the lookup runs with alpha 1.
+template <class R>
+void warpSliceVols(const std::vector<const V4<R> *> &vols, const Dims &Dv, R *dst, int dx,
+                   int dy, int dz, const R *phi, const Dims &Dp, R time, R blendAlpha,
+                   R loadTimeScale, const V4<R> &defoOffset, const V4<R> &defoScale,
+                   const V4<R> &defoFactor, const V4<R> &overrideSize, R overrideTimeOff,
+                   bool zeroVel, int bordSkip, R defoAniFac, bool doAligned,
+                   R partialLoadFac, R thirdAlpha, R fourthAlpha)
+{
+    V4<R> sf, off2 = defoOffset;
+    int ds[4] = {Dp.X, Dp.Y, Dp.Z, Dp.T};
+    if (overrideSize.x > 0.) {
+        ds[0] = (int)overrideSize.x;
+        ds[1] = (int)overrideSize.y;
+        ds[2] = (int)overrideSize.z;
+        ds[3] = (int)overrideSize.t;
+    }
+    gridFactor4d<R>(V4<R>((R)Dv.X, (R)Dv.Y, (R)Dv.Z, (R)Dv.T),
+                    V4<R>((R)ds[0], (R)ds[1], (R)ds[2], (R)ds[3]),
+                    V4<R>((R)-1.), defoScale, sf, off2);
+    const R srcTime = (R)(time * sf.t * loadTimeScale + off2.t - 0.5);
+    int t = (int)srcTime;
+    t = std::min(t, Dv.T - 1);
+    const R width = std::max((R)0.2, partialLoadFac);
+    const int W = (int)((R)Dv.T * width);
+    const int currt = W / 2;
+    const int dvOff = t - currt;
+    const Dims Dw{Dv.X, Dv.Y, Dv.Z, W};
+    const idx_t ns = Dv.st();
+    std::vector<std::vector<V4<R>>> win(vols.size(), std::vector<V4<R>>(ns * W));
+    for (size_t v = 0; v < vols.size(); ++v)
+        for (int tl = 0; tl < W; ++tl) {
+            const int it = std::min(std::max(t - currt + tl, 0), Dv.T - 1);
+            std::copy(vols[v] + ns * it, vols[v] + ns * (it + 1), win[v].begin() + ns * tl);
+        }
+    std::vector<V4<R>> vt(ns);
+    const R pt = srcTime - (R)dvOff;
+    const V4<R> h((R)0.5);
+    auto get = [&](size_t v, const V4<R> &q) {
+        return interp4<R, V4<R>>(win[v].data(), Dw, q.x, q.y, q.z, q.t);
+    };
+    if (vols.size() == 2 && !doAligned) {
+        for (int k = 0; k < Dv.Z; ++k)
+            for (int j = 0; j < Dv.Y; ++j)
+                for (int i = 0; i < Dv.X; ++i) {
+                    const V4<R> p((R)i, (R)j, (R)k, pt);
+                    const V4<R> v2 = get(1, p + h);
+                    vt[Dv.at(i, j, k, 0)] = get(0, p + h - v2) * blendAlpha + v2 * thirdAlpha;
+                }
+    } else if (vols.size() == 2) {
+        std::vector<V4<R>> dvt(ns * W);
+        for (int tt = 0; tt < W; ++tt)
+            for (int k = 0; k < Dv.Z; ++k)
+                for (int j = 0; j < Dv.Y; ++j)
+                    for (int i = 0; i < Dv.X; ++i) {
+                        const V4<R> p((R)i, (R)j, (R)k, (R)tt);
+                        const V4<R> v2 = get(1, p + h);
+                        dvt[Dw.at(i, j, k, tt)] =
+                            get(0, p + h - v2) * blendAlpha + v2 * thirdAlpha;
+                    }
+        const R back = (R)(1. - blendAlpha);
+        for (int k = 0; k < Dv.Z; ++k)
+            for (int j = 0; j < Dv.Y; ++j)
+                for (int i = 0; i < Dv.X; ++i) {
+                    const V4<R> p((R)i, (R)j, (R)k, pt);
+                    const V4<R> v1 = get(0, p + h) * (R)-1;
+                    const V4<R> q = p + h - v1 * back;
+                    vt[Dv.at(i, j, k, 0)] = interp4<R, V4<R>>(dvt.data(), Dw, q.x, q.y, q.z, q.t);
+                }
+    } else if (vols.size() == 3) {
+        for (int k = 0; k < Dv.Z; ++k)
+            for (int j = 0; j < Dv.Y; ++j)
+                for (int i = 0; i < Dv.X; ++i) {
+                    const V4<R> p((R)i, (R)j, (R)k, pt);
+                    const V4<R> v3 = get(2, p + h);
+                    const V4<R> v2 = get(1, p + h - v3);
+                    const V4<R> v1 = get(0, p + h - v3 - v2);
+                    vt[Dv.at(i, j, k, 0)] =
+                        v1 * blendAlpha + v2 * thirdAlpha + v3 * fourthAlpha;
+                }
+    } else {
+        throw std::runtime_error("Code currently only supports 2 deformation volumes");
+    }
+    if (zeroVel) std::fill(vt.begin(), vt.end(), V4<R>(0));
+    lookupSlice<R>(vt.data(), Dims{Dv.X, Dv.Y, Dv.Z, 1}, dst, dx, dy, dz, phi, Dp,
+                   time + overrideTimeOff, (R)1, sf, off2, defoFactor * V4<R>(defoAniFac),
+                   bordSkip, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1464,6 +1568,34 @@ using namespace orc;
                      (R)loadTimeScale, v4(defoOffset), v4(defoScale),          \
                      v4(defoFactor), v4(overrideSize), (R)overrideTimeOff,     \
                      zeroVel != 0, bordSkip, (R)defoAniFac, optimized != 0);   \
+    }                                                                          \
+    extern "C" int orc_warp_slice_vols##SUF(                                   \
+        const R *const *vols, int nvols, int vX, int vY, int vZ, int vT,       \
+        R *dst, int dx, int dy, int dz, const R *phi, int pX, int pY, int pZ,  \
+        int pT, double time, double blendAlpha, double loadTimeScale,          \
+        const double *defoOffset, const double *defoScale,                     \
+        const double *defoFactor, const double *overrideSize,                  \
+        double overrideTimeOff, int zeroVel, int bordSkip, double defoAniFac,  \
+        int doAligned, double partialLoadFac, double thirdAlpha,               \
+        double fourthAlpha)                                                    \
+    {                                                                          \
+        auto v4 = [](const double *a) {                                        \
+            return V4<R>((R)a[0], (R)a[1], (R)a[2], (R)a[3]);                  \
+        };                                                                     \
+        std::vector<const V4<R> *> vv;                                         \
+        for (int q = 0; q < nvols; ++q) vv.push_back((const V4<R> *)vols[q]);  \
+        try {                                                                  \
+            warpSliceVols<R>(vv, Dims{vX, vY, vZ, vT}, dst, dx, dy, dz, phi,   \
+                             Dims{pX, pY, pZ, pT}, (R)time, (R)blendAlpha,     \
+                             (R)loadTimeScale, v4(defoOffset), v4(defoScale),  \
+                             v4(defoFactor), v4(overrideSize),                 \
+                             (R)overrideTimeOff, zeroVel != 0, bordSkip,       \
+                             (R)defoAniFac, doAligned != 0, (R)partialLoadFac, \
+                             (R)thirdAlpha, (R)fourthAlpha);                   \
+        } catch (const std::exception &) {                                     \
+            return -1;                                                         \
+        }                                                                      \
+        return 0;                                                              \
     }                                                                          \
     extern "C" void orc_repeat_frame4d##SUF(R *phi, int X, int Y, int Z, int T,  \
                                             double srct, double range,         \

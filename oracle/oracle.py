@@ -113,6 +113,9 @@ def _declare(L):
             "orc_warp_slice": ([_P, _I, _I, _I, _I, _P, _I, _I, _I, _P, _I, _I,
                                 _I, _I, _D, _D, _D, _P, _P, _P, _P, _D, _I, _I,
                                 _D, _I], None),
+            "orc_warp_slice_vols": ([_P, _I, _I, _I, _I, _I, _P, _I, _I, _I, _P,
+                                     _I, _I, _I, _I, _D, _D, _D, _P, _P, _P, _P,
+                                     _D, _I, _I, _D, _I, _D, _D, _D], _I),
             "orc_drop_sdf": ([_P, _I, _I, _I, _I, _I, C.c_uint64, _I, _D, _D,
                               _D], None),
             "orc_repeat_frame4d": ([_P, _I, _I, _I, _I, _D, _D, _I], None),
@@ -366,6 +369,31 @@ def warp_slice(defo, phi, dst, time, blendAlpha=1.0, loadTimeScale=1.0,
         time, blendAlpha, loadTimeScale, _v4(defoOffset), _v4(defoScale),
         _v4(defoFactor), _v4(overrideSize), overrideTimeOff, int(zeroVel),
         bordSkip, defoAniFac, int(optimized))
+    return d
+
+
+def warp_slice_vols(vols, phi, dst, time, blendAlpha=1.0, loadTimeScale=1.0,
+                    defoOffset=(0, 0, 0, 0), defoScale=(1, 1, 1, 1),
+                    defoFactor=(1, 1, 1, 1), overrideSize=(-1, -1, -1, -1),
+                    overrideTimeOff=0.0, zeroVel=False, bordSkip=1, defoAniFac=1.0,
+                    doAligned=False, partialLoadFac=0.2, thirdAlpha=0.0,
+                    fourthAlpha=0.0):
+    """loadAdvectTimeSlice_OptRun with deformation volumes (useDefoVols,
+    optflow4d.cpp:1786-1870, 2013-2085): vols is a list of 2 or 3 defos
+    (T, Z, Y, X, 4) of one size; returns the updated copy of dst."""
+    vols = [np.ascontiguousarray(v) for v in vols]
+    dtype = vols[0].dtype
+    phi = np.ascontiguousarray(phi, dtype=dtype)
+    d = np.array(dst, copy=True, order="C", dtype=dtype)
+    dz, dy, dx = d.shape
+    arr = (C.c_void_p * len(vols))(*[v.ctypes.data for v in vols])
+    rc = _fn("orc_warp_slice_vols", vols[0])(
+        arr, len(vols), *_dims(vols[0]), _ptr(d), dx, dy, dz, _ptr(phi), *_dims(phi),
+        time, blendAlpha, loadTimeScale, _v4(defoOffset), _v4(defoScale),
+        _v4(defoFactor), _v4(overrideSize), overrideTimeOff, int(zeroVel), bordSkip,
+        defoAniFac, int(doAligned), partialLoadFac, thirdAlpha, fourthAlpha)
+    if rc != 0:
+        raise RuntimeError("warp_slice_vols: 2 or 3 deformation volumes")
     return d
 
 

@@ -176,3 +176,50 @@ def test_advect_time_slice_non_cached(tmp_path):
         ref = O.warp_slice(dv, ref_phi, dst0, time, blendAlpha=0.8,
                            defoFactor=(2, 2, 2, 1), optimized=False)
         assert np.array_equal(dst.to_numpy(), ref)
+
+
+@pytest.mark.parametrize("nv,aligned", [(2, False), (3, False), (2, True)])
+def test_advect_time_slice_defo_volumes(tmp_path, nv, aligned):
+    """loadAdvectTimeSlice_OptInit(useDefoVols) / _OptAdd / _OptRun: the
+    windowed blend of 2 or 3 deformation volumes equals the oracle bit for bit
+    (grid entry point and file-based handle)."""
+    say("defo volumes", nv, "aligned", aligned)
+    N, T = 24, 12
+    phi0 = RNG.standard_normal((T, N, N, N)).astype(np.float32)
+    vols = [(RNG.standard_normal((T, N, N, N, 4)) * 0.6).astype(np.float32)
+            for _ in range(nv)]
+    s = solver(N, N, N, T)
+    phi = real(s, phi0)
+    gv = []
+    for v in vols:
+        g = s.create(M.Grid4Vec4)
+        g.from_numpy(v)
+        gv.append(g)
+    dst0 = RNG.standard_normal((N, N, N)).astype(np.float32)
+    kw = dict(blendAlpha=0.7, thirdAlpha=0.4, fourthAlpha=0.2 if nv == 3 else 0.0)
+    for time in (0.0, 5.3, T - 1.0):
+        dst = M.DeviceArray.from_numpy(dst0)
+        M.loadAdvectTimeSliceFromDefoVols(gv, dst, phi, time, kw["blendAlpha"],
+                                          thirdAlpha=kw["thirdAlpha"],
+                                          fourthAlpha=kw["fourthAlpha"], bordSkip=0,
+                                          doAligned=aligned, partialLoadFac=0.3)
+        ref = O.warp_slice_vols(vols, phi0, dst0, time, bordSkip=0, doAligned=aligned,
+                                partialLoadFac=0.3, **kw)
+        assert np.array_equal(dst.to_numpy(), ref), time
+    # the same through the file-based handle
+    for q, v in enumerate(vols):
+        M.writeUni4d(tmp_path / ("d%d.uni" % q), v)
+    M.loadAdvectTimeSlice_OptInit(3, str(tmp_path / "d0.uni"), useDefoVols=True,
+                                  doAligned=aligned, partialLoadFac=0.3)
+    try:
+        for q in range(1, nv):
+            M.loadAdvectTimeSlice_OptAdd(3, str(tmp_path / ("d%d.uni" % q)))
+        dst = M.DeviceArray.from_numpy(dst0)
+        M.loadAdvectTimeSlice_OptRun(3, "", dst, phi, 5.3, kw["blendAlpha"], 1.0, M.Vec4(0.),
+                                     M.Vec4(1.), M.Vec4(1.), thirdAlpha=kw["thirdAlpha"],
+                                     bordSkip=0, fourthAlpha=kw["fourthAlpha"])
+        ref = O.warp_slice_vols(vols, phi0, dst0, 5.3, bordSkip=0, doAligned=aligned,
+                                partialLoadFac=0.3, **kw)
+        assert np.array_equal(dst.to_numpy(), ref)
+    finally:
+        M.loadAdvectTimeSlice_Finish(3)

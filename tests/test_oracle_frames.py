@@ -1,6 +1,7 @@
 """Pins the oracle's frame placement (loadPlaceGrid4d), frame warp
 (loadAdvectTimeSlice[_OptRun]) and the seeded drop generator (no GPU)."""
 import numpy as np
+import pytest
 from scipy.ndimage import map_coordinates
 
 from oracle import oracle as O
@@ -196,3 +197,69 @@ def test_extrap_planar_closed_form():
     out_in = O.extrap4d_ls_simple(phi, distance=dist, inside=True)
     first_in = phi[core + (3,)]
     np.testing.assert_allclose(out_in[core + (2,)], first_in - 1, atol=1e-12)
+
+
+def _window(v, time, W):
+    """The reference's defo window of optflow4d.cpp:1822-1866 for srcTime."""
+    T = v.shape[0]
+    t = min(int(time), T - 1)
+    cur = W // 2
+    idx = [min(max(t - cur + tl, 0), T - 1) for tl in range(W)]
+    return v[idx], time - (t - cur)
+
+
+def mc4v(vol, pos):
+    """map_coordinates of a (T,Z,Y,X,4) field at positions (..., 4) (x,y,z,t),
+    cell centres at c + 1/2, edge clamp."""
+    coords = np.stack([pos[..., 3] - 0.5, pos[..., 2] - 0.5, pos[..., 1] - 0.5,
+                       pos[..., 0] - 0.5])
+    return np.stack([map_coordinates(vol[..., c], coords, order=1, mode="nearest")
+                     for c in range(4)], -1)
+
+
+def test_warp_defo_volumes_against_map_coordinates():
+    """Two volumes (not aligned): vt = alpha dv1(p - dv2(p)) + third dv2(p) on
+    the clamped window; three volumes chain; aligned: dvt on the whole window,
+    then vt = dvt(p + (1 - alpha) dv1(p)); the lookup then runs with alpha 1
+    over the bnd-10 interior.  Restated with map_coordinates on windows built
+    here."""
+    N, T = 24, 10
+    time = 4.3
+    phi = RNG.standard_normal((T, N, N, N))
+    vols = [RNG.standard_normal((T, N, N, N, 4)) * 0.6 for _ in range(3)]
+    dst0 = np.zeros((N, N, N))
+    k, j, i = np.meshgrid(*[np.arange(N)] * 3, indexing="ij")
+    W = int(T * 0.2)
+    wins = []
+    for v in vols:
+        wv, pt = _window(v, time, W)   # srcTime = time (sf_t 1, off_t 1/2)
+        wins.append(wv)
+    p = np.stack([i + .5, j + .5, k + .5, np.full(i.shape, pt + .5)], -1)
+    pos0 = np.stack([i + .5, j + .5, k + .5, np.full(i.shape, time + .5)], -1)
+    b = (slice(10, -10),) * 3
+
+    def check(out, vt):
+        np.testing.assert_allclose(out[b], mc4(phi, pos0 - vt)[b], atol=1e-9)
+        mask = np.ones(out.shape, bool)
+        mask[b] = False
+        assert not out[mask].any()    # outside the bnd-10 interior: untouched
+
+    out = O.warp_slice_vols(vols[:2], phi, dst0, time, blendAlpha=0.7, thirdAlpha=0.4,
+                            bordSkip=0)
+    v2 = mc4v(wins[1], p)
+    check(out, 0.7 * mc4v(wins[0], p - v2) + 0.4 * v2)
+    out = O.warp_slice_vols(vols, phi, dst0, time, blendAlpha=0.5, thirdAlpha=0.3,
+                            fourthAlpha=0.2, bordSkip=0)
+    v3 = mc4v(wins[2], p)
+    v2 = mc4v(wins[1], p - v3)
+    v1 = mc4v(wins[0], p - v3 - v2)
+    check(out, 0.5 * v1 + 0.3 * v2 + 0.2 * v3)
+    out = O.warp_slice_vols(vols[:2], phi, dst0, time, blendAlpha=0.7, thirdAlpha=0.4,
+                            bordSkip=0, doAligned=True)
+    kk, jj, ii, tt = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), np.arange(W),
+                                 indexing="ij")
+    q = np.stack([ii + .5, jj + .5, kk + .5, tt + .5], -1).transpose(3, 0, 1, 2, 4)
+    dvt = 0.7 * mc4v(wins[0], q - mc4v(wins[1], q)) + 0.4 * mc4v(wins[1], q)
+    check(out, mc4v(dvt, p + 0.3 * mc4v(wins[0], p)))
+    with pytest.raises(RuntimeError):
+        O.warp_slice_vols(vols[:1], phi, dst0, time)
