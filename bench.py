@@ -6,9 +6,12 @@ A step is one device-resident multiscale FlOF solve at 64^4 (seeded drop
 SDFs, wSmooth 0.1, wEnergy 1e-4, 3 levels, every level run to relative
 signed-max residual 1e-5 -- the iteration cap is raised above the
 reference's 1000 so the residual is the same at every level).  The JSON line
-reports solves/s, the HBM roofline of the dominant kernel (k_cg_update_tm),
-timed live with HIP events on the library's stream, and the CPU oracle timed
-on a bounded sample of the same workload on this host.
+reports solves/s of K solves on the production launch path (PCG iterations
+as captured HIP graphs), the HBM roofline of the dominant kernel (the PCG
+update) from the same K solves repeated with a HIP event pair on every PCG
+launch on the library's stream (events force plain launches, which costs a
+few percent of the solve, so they are kept out of the timed solves), and the
+CPU oracle timed on a bounded sample of the same workload on this host.
 """
 import argparse
 import ctypes as C
@@ -35,6 +38,12 @@ B_UPDATE = 16 * 3 + 4 + 16 * 2   # read p,r,x,i1; write r,x
 B_APPLY = 16 + 4                 # read p,i1 (p.Ap only)
 B_DIR = 16 * 2 + 4 + 16          # read r,p,i1; write p
 B_ITER_MIN = 112                 # x,r,p read+written once, g read once
+# kernel names of the PCG launch kinds: the LDS-tiled stencil kernels are the
+# library's default (FLOF_LDS_STENCIL=0 selects the t-marching ones)
+LDS_STENCIL = os.environ.get("FLOF_LDS_STENCIL", "1")[:1] == "1"
+KN_UPDATE = "k_cg_stencil_lds<update>" if LDS_STENCIL else "k_cg_update_tm"
+KN_APPLY = "k_cg_stencil_lds<pq>" if LDS_STENCIL else "k_cg_pq_tm"
+KN_DIR = "k_cg_dir_tm"
 
 
 def log(*a):
@@ -110,10 +119,7 @@ def main():
     for _ in range(args.warmup):
         solve(FIXED_RESIDUAL_CAP)
     M.sync()
-    kinds = (FLOF_KT_CG_APPLY, FLOF_KT_CG_UPDATE, FLOF_KT_CG_DIR)
-    for k in kinds:
-        L.flof_ctx_timing(ctx, k, None, None)  # drop earlier records
-        L.flof_ctx_enable_timing(ctx, k, 1)
+    # timed region: K solves on the production launch path
     times = []
     infos = []
     for st in range(args.steps):
@@ -125,6 +131,21 @@ def main():
         infos.append(M.getLastSolveInfo().summary())
         log("bench: step %d %.3f s iters %s" % (
             st, times[-1], [l["iters"][0] for l in infos[-1]["levels"]]))
+    # the same K solves with a HIP event pair on every PCG launch: the
+    # per-kernel durations of the roofline
+    kinds = (FLOF_KT_CG_APPLY, FLOF_KT_CG_UPDATE, FLOF_KT_CG_DIR)
+    for k in kinds:
+        L.flof_ctx_timing(ctx, k, None, None)  # drop earlier records
+        L.flof_ctx_enable_timing(ctx, k, 1)
+    t_events = []
+    for st in range(args.steps):
+        M.sync()
+        t0 = time.perf_counter()
+        solve(FIXED_RESIDUAL_CAP)
+        M.sync()
+        t_events.append(time.perf_counter() - t0)
+        it_ev = [l["iters"][0] for l in M.getLastSolveInfo().summary()["levels"]]
+        log("bench: step %d with kernel events %.3f s iters %s" % (st, t_events[-1], it_ev))
     cells0 = N ** 4
     ktime, ktime0 = {}, {}
     for k in kinds:
@@ -141,15 +162,6 @@ def main():
     cell_iters = sum(i * c for i, c in zip(iters, cells))
     ms_step = 1000.0 * float(np.median(times))
     solves_per_s = 1000.0 / ms_step
-    # the same solves without per-launch events (iterations enqueued as
-    # captured graphs): the production launch path, for comparison
-    t_plain = []
-    for _ in range(2):
-        M.sync()
-        t0 = time.perf_counter()
-        solve(FIXED_RESIDUAL_CAP)
-        M.sync()
-        t_plain.append(time.perf_counter() - t0)
 
     # all levels: algorithmic bytes of all real launches over the summed
     # event time of every launch of that kernel
@@ -170,8 +182,8 @@ def main():
                 "avg_us_per_iteration": 1000.0 * ms / max(1, real),
                 "bytes_per_launch": bpc * cells0,
                 "GBps": byts / (ms * 1e-3) / 1e9 if ms > 0 else None}
-    names = {FLOF_KT_CG_UPDATE: ("k_cg_update_tm", B_UPDATE),
-             FLOF_KT_CG_APPLY: ("k_cg_pq_tm", B_APPLY), FLOF_KT_CG_DIR: ("k_cg_dir_tm", B_DIR)}
+    names = {FLOF_KT_CG_UPDATE: (KN_UPDATE, B_UPDATE),
+             FLOF_KT_CG_APPLY: (KN_APPLY, B_APPLY), FLOF_KT_CG_DIR: (KN_DIR, B_DIR)}
     kst = {names[k][0]: kstat(k, names[k][1]) for k in kinds}
     kst0 = {names[k][0]: kstat0(k, names[k][1]) for k in kinds}
     dom = max(kst0, key=lambda k: kst0[k]["ms_total"])
@@ -182,8 +194,9 @@ def main():
                 "unit": "GB/s", "frac": achieved / HBM_PEAK,
                 "traffic": traffic,
                 "kernel": dom, "launch": "finest level, %d cells" % cells0,
-                "bytes_per_cell": {"k_cg_update_tm": B_UPDATE, "k_cg_pq_tm": B_APPLY,
-                                   "k_cg_dir_tm": B_DIR}[dom],
+                "bytes_per_cell": {KN_UPDATE: B_UPDATE, KN_APPLY: B_APPLY,
+                                   KN_DIR: B_DIR}[dom],
+                "timed_over": "the K solves repeated with per-launch HIP events",
                 "avg_launch_us": kst0[dom]["avg_us_per_iteration"],
                 "pcg_iteration_us": 1000.0 * iter_ms0,
                 "pcg_iteration_frac_112B": (B_ITER_MIN * cells0 / (iter_ms0 * 1e-3)) / HBM_PEAK}
@@ -202,7 +215,7 @@ def main():
         "roofline": roofline,
         "kernels_finest_level": kst0,
         "kernels_all_levels": kst,
-        "ms_per_step_without_kernel_events": 1000.0 * min(t_plain),
+        "ms_per_step_with_kernel_events": 1000.0 * float(np.median(t_events)),
     }
 
     # host-buffer boundary: the inputs come from and the flow goes to host

@@ -149,6 +149,13 @@ int flof_ctx_timing_tag(flof_ctx *ctx, int kind, int64_t tag, int64_t *launches,
 int flof_ctx_set_autotune(flof_ctx *ctx, int enable);
 int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T);
 
+/* Kernels of the PCG's two stencil passes, p.(A p) and the update (no
+ * reference counterpart; results are identical): FLOF_STENCIL_LDS (the
+ * default) tiles 64x4x4 cells in LDS, FLOF_STENCIL_TMARCH reads the
+ * neighbours through the caches. */
+enum flof_stencil { FLOF_STENCIL_TMARCH = 0, FLOF_STENCIL_LDS = 1 };
+int flof_ctx_set_stencil(flof_ctx *ctx, int kind);
+
 /* ---- Grid4d storage (source/grid4d.h:117-265, source/grid4d.cpp:60-138) - */
 /* Creates a zero-initialised X*Y*Z*T grid (x fastest, then y, z, t). */
 int flof_grid4d_create(flof_ctx *ctx, int X, int Y, int Z, int T, int elem,

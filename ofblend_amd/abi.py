@@ -19,6 +19,9 @@ FLOF_MAX_STEPS = 10
 FLOF_PSTORE_F32 = 0
 FLOF_PSTORE_BF16 = 1
 
+FLOF_STENCIL_TMARCH = 0
+FLOF_STENCIL_LDS = 1
+
 FLOF_KT_CG_APPLY = 0
 FLOF_KT_CG_UPDATE = 1
 FLOF_KT_CG_DIR = 2

@@ -851,6 +851,12 @@ int flof_ctx_create(flof_ctx **out, int device)
         if (bp && std::atoi(bp) > 0 && std::atoi(bp) <= 64) ctx->blocks_per_cu = std::atoi(bp);
         const char *nt = std::getenv("FLOF_NO_TMARCH");
         ctx->use_tmarch = !(nt && nt[0] == '1');
+        const char *ls = std::getenv("FLOF_LDS_STENCIL");
+        if (ls) ctx->lds_stencil = ls[0] == '1';
+        const char *lx = std::getenv("FLOF_LDS_XCD");
+        if (lx) ctx->lds_xcd = lx[0] == '1';
+        const char *lb = std::getenv("FLOF_LDS_BPC");
+        if (lb && std::atoi(lb) > 0 && std::atoi(lb) <= 64) ctx->lds_blocks_per_cu = std::atoi(lb);
         const char *tb = std::getenv("FLOF_TM_BPC");
         if (tb && std::atoi(tb) > 0 && std::atoi(tb) <= 64) ctx->tm_blocks_per_cu = std::atoi(tb);
     }
@@ -894,10 +900,19 @@ int flof_ctx_set_autotune(flof_ctx *ctx, int enable)
     return FLOF_OK;
 }
 
+int flof_ctx_set_stencil(flof_ctx *ctx, int kind)
+{
+    if (!ctx || (kind != FLOF_STENCIL_TMARCH && kind != FLOF_STENCIL_LDS)) return FLOF_ERR_ARG;
+    ctx->lds_stencil = kind == FLOF_STENCIL_LDS;
+    for (flof_ctx *w : ctx->workers) w->lds_stencil = ctx->lds_stencil;
+    return FLOF_OK;
+}
+
 int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T)
 {
     if (!ctx) return FLOF_ERR_ARG;
-    auto it = ctx->tm_tuned.find(std::array<int, 5>{X, Y, Z, T, ctx->p_store});
+    auto it = ctx->tm_tuned.find(
+        std::array<int, 5>{X, Y, Z, T, tune_variant(ctx)});
     return it == ctx->tm_tuned.end() ? 0 : it->second;
 }
 

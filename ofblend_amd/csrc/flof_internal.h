@@ -84,6 +84,12 @@ struct flof_ctx {
     // CU (FLOF_TM_BPC)
     bool use_tmarch = true;
     int tm_blocks_per_cu = 4;
+    // LDS-tiled stencil kernels for p.Ap and the update (FLOF_LDS_STENCIL=0
+    // selects the t-marching ones), their work items per CU (FLOF_LDS_BPC)
+    // and XCD-contiguous work items (FLOF_LDS_XCD=0 disables)
+    bool lds_stencil = true;
+    int lds_blocks_per_cu = 2;
+    bool lds_xcd = true;
     // device scratch for reductions (partials + state), grown on demand
     void *red = nullptr;
     size_t red_bytes = 0;
@@ -98,6 +104,12 @@ struct flof_ctx {
     // (FLOF_PSTORE_*), set from flof_of_params.precision by the entry points
     int p_store = FLOF_PSTORE_F32;
 };
+
+// the kernel variant part of the autotuner's key of a grid shape
+inline int tune_variant(const flof_ctx *ctx)
+{
+    return ctx->p_store + (ctx->lds_stencil ? 2 : 0);
+}
 
 struct flof_warp_state {
     flof_ctx *ctx = nullptr;

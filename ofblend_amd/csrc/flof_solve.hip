@@ -692,6 +692,7 @@ __device__ __forceinline__ void st_stream(float4 *a, const float4 &v)
 
 struct PStoreF32 {
     typedef float4 T;
+    static __device__ __forceinline__ float4 rnd(const float4 &v) { return v; }
     static __device__ __forceinline__ float4 ld(const T *p, int64_t c) { return p[c]; }
     static __device__ __forceinline__ void st(T *p, int64_t c, const float4 &v) { p[c] = v; }
 #if FLOF_STREAM_NT >= 2
@@ -725,6 +726,12 @@ __device__ __forceinline__ unsigned short f32_to_bf16(float f)
 }
 struct PStoreBF16 {
     typedef bf4v T;
+    // the value a store followed by a load gives
+    static __device__ __forceinline__ float4 rnd(const float4 &v)
+    {
+        return make_float4(bf16_to_f32(f32_to_bf16(v.x)), bf16_to_f32(f32_to_bf16(v.y)),
+                           bf16_to_f32(f32_to_bf16(v.z)), bf16_to_f32(f32_to_bf16(v.w)));
+    }
     static __device__ __forceinline__ float4 ld(const T *p, int64_t c)
     {
         const bf4v h = p[c];
@@ -925,6 +932,281 @@ k_cg_dir_tm(typename PS::T *__restrict__ p, const float4 *__restrict__ r,
         PS::st_once(p, c, np);
         im = ic;
         ic = ip;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tiled variant of the two stencil kernels (k_cg_pq_tm, k_cg_update_tm).
+// A workgroup owns a tile of 64 (x) x 4 (y) x 4 (z) cells and marches it
+// through its t segment; thread (lane = x, wave = y) owns the column of 4
+// z cells.  Per t step the tile's p(t) and i1(t) go to LDS together with the
+// y-halo rows, x-halo columns and the z-halo planes; the t-1 / t+1 values of
+// the own cells stay in registers.  The halo of step t+1 and the own values
+// of t+2 are loaded while step t computes (the loads are waited for only
+// when they are written to LDS at the next step).  Every cell is computed
+// with the same operands in the same order as in the untiled kernels, so the
+// results are identical; only the neighbour loads change (one halo value per
+// own value instead of six neighbours through the caches).
+struct TileMarch {
+    int ntx, nty, ntz;  // tiles along x, y, z
+    int ntile;          // ntx * nty * ntz
+    int seglen, nseg;   // t segments
+    int total;          // ntile * nseg work items
+    int xcd;            // 1: the grid (a multiple of 8) gives each XCD a
+                        // contiguous range of work items
+};
+constexpr int LT_X = 64, LT_Y = 4, LT_Z = 4;
+constexpr int LS_X = LT_X + 2, LS_Y = LT_Y + 2;
+constexpr int LS_N = LT_Z * LS_Y * LS_X;  // 1584 entries: 4 z x 6 rows x 66
+
+// LDS slot of (z q in [0,4), row ry in [-1,4], column x in [-1,64])
+__device__ __forceinline__ int lslot(int q, int ry, int x)
+{
+    return (q * LS_Y + (ry + 1)) * LS_X + (x + 1);
+}
+
+__device__ __forceinline__ float4 f4scale(float s, const float4 &v)
+{
+    return make_float4(s * v.x, s * v.y, s * v.z, s * v.w);
+}
+
+template <class PS, bool UPDATE>
+__global__ void __launch_bounds__(256)
+k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__ i1,
+                 float4 *__restrict__ r, float4 *__restrict__ x, LevelGeom G, TileMarch M,
+                 float dxf, const CgState *st, DD *part_dd, float *part_max)
+{
+    if (st->done) return;  // uniform over the launch
+    // The neighbour sum of A p only ever uses offd * p(neighbour), so LDS
+    // holds those products (each formed once per cell instead of once per
+    // use; the same float products, so the sums are unchanged).
+    __shared__ float4 sp[LS_N];             // offd * p: tile, y-halo, x-halo
+    __shared__ float si[LS_N];              // i1 of the same cells
+    __shared__ float4 szp[2][LT_Y * LT_X];  // offd * p: z-halo planes z0-1, z0+4
+    __shared__ float szi[2][LT_Y * LT_X];
+    const Dims4 &D = G.D;
+    const float od = G.offd;
+    DD acc{0., 0.};
+    float mx = -INFINITY;
+    int L = blockIdx.x;
+    if (M.xcd) {
+        const int per = (int)gridDim.x / FLOF_NXCD;
+        L = (L % FLOF_NXCD) * per + L / FLOF_NXCD;
+    }
+    if (L < M.total) {  // uniform over the workgroup
+        const int seg = L / M.ntile;
+        const int tile = L - seg * M.ntile;
+        const int tx = tile % M.ntx, tyz = tile / M.ntx;
+        const int ty = tyz % M.nty, tz = tyz / M.nty;
+        const int x0 = tx * LT_X, y0 = ty * LT_Y, z0 = tz * LT_Z;
+        const int t0 = seg * M.seglen, t1 = min(D.T, t0 + M.seglen);
+        const int tid = (int)threadIdx.x;
+        const int lane = tid & 63, wy = tid >> 6;
+        const int i = x0 + lane, j = y0 + wy;
+        const bool in_xy = i < D.X && j < D.Y;
+        const bool sp_xy = i >= 1 && j >= 1 && i < D.X - 1 && j < D.Y - 1;
+        const int sz = (int)D.sz, stt = (int)D.st;
+        const int c3 = i + D.X * j;  // + sz * k (cell indices fit 31 bits)
+        bool own[LT_Z];
+#pragma unroll
+        for (int q = 0; q < LT_Z; ++q) own[q] = in_xy && z0 + q < D.Z;
+        const bool has_zm = in_xy && z0 >= 1;
+        const bool has_zp = in_xy && z0 + LT_Z < D.Z;
+        const int czm = c3 + sz * (z0 - 1), czp = c3 + sz * (z0 + LT_Z);
+        // y-halo rows y0-1 / y0+4 of the 4 z planes: 512 cells, 2 per thread
+        const int hy_side = tid & 1, hy_q = (tid >> 1) & 3, hy_lane = tid >> 3;
+        const int hy_row = hy_side ? LT_Y : -1;
+        bool hy_ok[2];
+        int hy_c[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int hi = x0 + hy_lane + 32 * h, hj = y0 + hy_row, hk = z0 + hy_q;
+            hy_ok[h] = hi < D.X && hj >= 0 && hj < D.Y && hk < D.Z;
+            hy_c[h] = hy_ok[h] ? hi + D.X * hj + sz * hk : 0;
+        }
+        // x-halo columns x0-1 / x0+64 of the 4 rows and 4 planes: threads 0..31
+        const int hx_side = tid & 1, hx_r = (tid >> 1) & 3, hx_q = (tid >> 3) & 3;
+        const int hx_col = hx_side ? LT_X : -1;
+        bool hx_ok;
+        int hx_c;
+        {
+            const int hi = x0 + hx_col, hj = y0 + hx_r, hk = z0 + hx_q;
+            hx_ok = tid < 32 && hi >= 0 && hi < D.X && hj < D.Y && hk < D.Z;
+            hx_c = hx_ok ? hi + D.X * hj + sz * hk : 0;
+        }
+        const double alpha = st->alpha, nalpha = -st->alpha;
+        const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        // halo of one t step in registers until it is written to LDS
+        float4 hy_v[2], hx_v, zm_v, zp_v;
+        float hy_i[2], hx_i, zm_i, zp_i;
+        auto load_halo = [&](int t) {
+            const int ct = stt * t;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                hy_v[h] = hy_ok[h] ? PS::ld(p, hy_c[h] + ct) : zero4;
+                hy_i[h] = hy_ok[h] ? i1[hy_c[h] + ct] : 0.f;
+            }
+            hx_v = hx_ok ? PS::ld(p, hx_c + ct) : zero4;
+            hx_i = hx_ok ? i1[hx_c + ct] : 0.f;
+            zm_v = has_zm ? PS::ld(p, czm + ct) : zero4;
+            zm_i = has_zm ? i1[czm + ct] : 0.f;
+            zp_v = has_zp ? PS::ld(p, czp + ct) : zero4;
+            zp_i = has_zp ? i1[czp + ct] : 0.f;
+        };
+        // own cells: offd * p at t-1; p at t, t+1 and (prefetched) t+2
+        float4 pms[LT_Z], pc[LT_Z], pp[LT_Z], pn[LT_Z];
+        float im[LT_Z], ic[LT_Z], ip[LT_Z], in[LT_Z];
+#pragma unroll
+        for (int q = 0; q < LT_Z; ++q) {
+            const int c = c3 + sz * (z0 + q) + stt * t0;
+            const bool hm = own[q] && t0 > 0, hp = own[q] && t0 + 1 < D.T;
+            pms[q] = hm ? f4scale(od, PS::ld(p, c - stt)) : zero4;
+            im[q] = hm ? i1[c - stt] : 0.f;
+            pc[q] = own[q] ? PS::ld(p, c) : zero4;
+            ic[q] = own[q] ? i1[c] : 0.f;
+            pp[q] = hp ? PS::ld(p, c + stt) : zero4;
+            ip[q] = hp ? i1[c + stt] : 0.f;
+        }
+        load_halo(t0);
+        const int wl = wy * LT_X + lane;
+        for (int t = t0; t < t1; ++t) {
+            const int ct = stt * t;
+#pragma unroll
+            for (int q = 0; q < LT_Z; ++q) {
+                sp[lslot(q, wy, lane)] = f4scale(od, pc[q]);
+                si[lslot(q, wy, lane)] = ic[q];
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                sp[lslot(hy_q, hy_row, hy_lane + 32 * h)] = f4scale(od, hy_v[h]);
+                si[lslot(hy_q, hy_row, hy_lane + 32 * h)] = hy_i[h];
+            }
+            if (tid < 32) {
+                sp[lslot(hx_q, hx_r, hx_col)] = f4scale(od, hx_v);
+                si[lslot(hx_q, hx_r, hx_col)] = hx_i;
+            }
+            szp[0][wl] = f4scale(od, zm_v);
+            szi[0][wl] = zm_i;
+            szp[1][wl] = f4scale(od, zp_v);
+            szi[1][wl] = zp_i;
+            // the streamed x and r of this step first (waited for in the
+            // compute), then the prefetch of the next step
+            float4 xo[LT_Z], ro[LT_Z];
+            if (UPDATE) {
+#pragma unroll
+                for (int q = 0; q < LT_Z; ++q) {
+                    const int c = c3 + sz * (z0 + q) + ct;
+                    xo[q] = own[q] ? ld_stream(x + c) : zero4;
+                    ro[q] = own[q] ? ld_stream(r + c) : zero4;
+                }
+            }
+            if (t + 1 < t1) {
+                load_halo(t + 1);
+#pragma unroll
+                for (int q = 0; q < LT_Z; ++q) {
+                    const bool hn = own[q] && t + 2 < D.T;
+                    const int c = c3 + sz * (z0 + q) + ct + 2 * stt;
+                    pn[q] = hn ? PS::ld(p, c) : zero4;
+                    in[q] = hn ? i1[c] : 0.f;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < LT_Z; ++q) {
+                if (!own[q]) continue;
+                const int k = z0 + q;
+                float4 qc, pr;
+                if (sp_xy && k >= 1 && k < D.Z - 1 && t >= 1 && t < D.T - 1) {
+                    float4 v = zero4;
+                    if (G.use_offd) {
+                        // offd * p of the neighbours in the order t-1, z-1, y-1,
+                        // x-1, x+1, y+1, z+1, t+1
+                        const float4 nb[8] = {
+                            pms[q],
+                            q > 0 ? sp[lslot(q > 0 ? q - 1 : 0, wy, lane)] : szp[0][wl],
+                            sp[lslot(q, wy - 1, lane)],
+                            sp[lslot(q, wy, lane - 1)],
+                            sp[lslot(q, wy, lane + 1)],
+                            sp[lslot(q, wy + 1, lane)],
+                            q < LT_Z - 1 ? sp[lslot(q < LT_Z - 1 ? q + 1 : 0, wy, lane)]
+                                         : szp[1][wl],
+                            f4scale(od, pp[q])};
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) {
+                            v.x = v.x + nb[m].x;
+                            v.y = v.y + nb[m].y;
+                            v.z = v.z + nb[m].z;
+                            v.w = v.w + nb[m].w;
+                        }
+                    }
+                    float4 gc;
+                    gc.x = (si[lslot(q, wy, lane + 1)] - si[lslot(q, wy, lane - 1)]) * dxf;
+                    gc.y = (si[lslot(q, wy + 1, lane)] - si[lslot(q, wy - 1, lane)]) * dxf;
+                    {
+                        const float izn = q > 0 ? ic[q > 0 ? q - 1 : 0] : szi[0][wl];
+                        const float izq = q < LT_Z - 1 ? ic[q < LT_Z - 1 ? q + 1 : 0] : szi[1][wl];
+                        gc.z = (izq - izn) * dxf;
+                    }
+                    gc.w = (ip[q] - im[q]) * dxf;
+                    const float gg[4] = {gc.x, gc.y, gc.z, gc.w};
+                    const float pv[4] = {pc[q].x, pc[q].y, pc[q].z, pc[q].w};
+                    float o[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+#pragma unroll
+                        for (int m = 0; m < 4; ++m) {
+                            float b = gg[d] * gg[m];
+                            if (d == m) b = b + G.diag;
+                            o[d] = o[d] + b * pv[m];
+                        }
+                    }
+                    qc = make_float4(o[0], o[1], o[2], o[3]);
+                    if (UPDATE) pr = prec_f32(gc, G.diag);
+                } else {
+                    qc = apply_I(pc[q]);
+                    if (UPDATE) pr = make_float4(1.f, 1.f, 1.f, 1.f);
+                }
+                if (!UPDATE) {
+                    dd_add4(acc, pc[q], qc);
+                } else {
+                    const int c = c3 + sz * k + ct;
+                    float4 xc = xo[q];
+                    xc.x = (float)((double)xc.x + alpha * (double)pc[q].x);
+                    xc.y = (float)((double)xc.y + alpha * (double)pc[q].y);
+                    xc.z = (float)((double)xc.z + alpha * (double)pc[q].z);
+                    xc.w = (float)((double)xc.w + alpha * (double)pc[q].w);
+                    st_stream(x + c, xc);
+                    float4 rc = ro[q];
+                    rc.x = (float)((double)rc.x + nalpha * (double)qc.x);
+                    rc.y = (float)((double)rc.y + nalpha * (double)qc.y);
+                    rc.z = (float)((double)rc.z + nalpha * (double)qc.z);
+                    rc.w = (float)((double)rc.w + nalpha * (double)qc.w);
+                    st_stream(r + c, rc);
+                    const float4 z = f4mul(rc, pr);
+                    mx = fmaxf(mx, fmaxf(fmaxf(rc.x, rc.y), fmaxf(rc.z, rc.w)));
+                    dd_add4(acc, z, rc);
+                }
+            }
+            // offd * p(t) of the own cells is the t-1 neighbour of the next
+            // step (the thread's own LDS slots: no barrier needed)
+#pragma unroll
+            for (int q = 0; q < LT_Z; ++q) pms[q] = sp[lslot(q, wy, lane)];
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < LT_Z; ++q) {
+                pc[q] = pp[q];
+                pp[q] = pn[q];
+                im[q] = ic[q];
+                ic[q] = ip[q];
+                ip[q] = in[q];
+            }
+        }
+    }
+    acc = block_dd(acc);
+    if (UPDATE) mx = block_max(mx);
+    if (threadIdx.x == 0) {
+        part_dd[blockIdx.x] = acc;
+        if (UPDATE) part_max[blockIdx.x] = mx;
     }
 }
 
@@ -1212,6 +1494,24 @@ static TMarch make_tmarch(flof_ctx *ctx, const Dims4 &D, int blocks_per_cu)
     return M;
 }
 
+static TileMarch make_tilemarch(flof_ctx *ctx, const Dims4 &D, int blocks_per_cu)
+{
+    TileMarch M;
+    M.ntx = (D.X + LT_X - 1) / LT_X;
+    M.nty = (D.Y + LT_Y - 1) / LT_Y;
+    M.ntz = (D.Z + LT_Z - 1) / LT_Z;
+    M.ntile = M.ntx * M.nty * M.ntz;
+    const int target = ctx->num_cu * blocks_per_cu;
+    int nseg = (target + M.ntile - 1) / M.ntile;
+    if (nseg < 1) nseg = 1;
+    if (nseg > D.T) nseg = D.T;
+    M.seglen = (D.T + nseg - 1) / nseg;
+    M.nseg = (D.T + M.seglen - 1) / M.seglen;
+    M.total = M.ntile * M.nseg;
+    M.xcd = ctx->lds_xcd ? 1 : 0;
+    return M;
+}
+
 // candidates of the t-march autotuner (workgroups per CU)
 static const int TM_CANDIDATES[] = {2, 4, 8, 16};
 
@@ -1219,13 +1519,21 @@ struct LevelWork {
     Tmp g, prec, x, r, p, q, st, part;
     int nb = 0;     // grid of the streaming kernels
     int nb_tm = 0;  // grid of the t-marching kernels
+    int nb_lds = 0; // grid of the LDS-tiled stencil kernels
     int cap = 0;    // partial slots
     TMarch tm;
+    TileMarch tl;
     void set_tm(flof_ctx *ctx, const Dims4 &D, int bpc)
     {
         tm = make_tmarch(ctx, D, bpc);
         nb_tm = ((tm.total + FLOF_NXCD - 1) / FLOF_NXCD) * FLOF_NXCD;
+        tl = make_tilemarch(ctx, D, ctx->lds_blocks_per_cu);
+        nb_lds = tl.xcd ? ((tl.total + FLOF_NXCD - 1) / FLOF_NXCD) * FLOF_NXCD : tl.total;
+        // the LDS kernels index cells (and the t+2 prefetch) in 32 bits
+        lds = ctx->lds_stencil && D.st * (int64_t)(D.T + 2) < (1ll << 31);
     }
+    bool lds = false;  // p.Ap and the update run as the LDS-tiled kernels
+    int nb_stencil() const { return lds ? nb_lds : nb_tm; }
     LevelWork(flof_ctx *ctx, const Dims4 &D)
     {
         const int64_t n = (int64_t)D.st * D.T;
@@ -1239,9 +1547,9 @@ struct LevelWork {
         nb = grid_blocks_xcd(ctx, n);
         // partial slots for the largest t-march grid the autotuner may pick
         set_tm(ctx, D, TM_CANDIDATES[3]);
-        cap = std::max(nb, nb_tm);
+        cap = std::max(nb, std::max(nb_tm, nb_lds));
         set_tm(ctx, D, ctx->tm_blocks_per_cu);
-        cap = std::max(cap, nb_tm);
+        cap = std::max(cap, std::max(nb_tm, nb_lds));
         st = Tmp(ctx, sizeof(CgState));
         part = Tmp(ctx, (size_t)cap * (3 * sizeof(DD) + sizeof(float)) + 256);
         FLOF_HIP(hipMemsetAsync(st.p, 0, sizeof(CgState), ctx->stream));
@@ -1260,17 +1568,27 @@ static void enqueue_tm_iter(flof_ctx *ctx, LevelWork &W, void *pbuf, const float
                             bool finalizers)
 {
     typedef typename PS::T PT;
-    klaunch(ctx, ka, k_cg_pq_tm<PS>, dim3(W.nb_tm), dim3(256), (const PT *)pbuf, i1, G, W.tm,
-            dxf, (const CgState *)st, W.pdd());
+    const int ns = W.nb_stencil();
+    if (W.lds)
+        klaunch(ctx, ka, k_cg_stencil_lds<PS, false>, dim3(ns), dim3(256), (const PT *)pbuf, i1,
+                (float4 *)nullptr, (float4 *)nullptr, G, W.tl, dxf, (const CgState *)st,
+                W.pdd(), (float *)nullptr);
+    else
+        klaunch(ctx, ka, k_cg_pq_tm<PS>, dim3(ns), dim3(256), (const PT *)pbuf, i1, G, W.tm,
+                dxf, (const CgState *)st, W.pdd());
     if (finalizers)
-        hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
-                           W.nb_tm);
-    klaunch(ctx, ku, k_cg_update_tm<PS>, dim3(W.nb_tm), dim3(256), W.r.as<float4>(),
-            W.x.as<float4>(), (const PT *)pbuf, i1, G, W.tm, dxf, (const CgState *)st, W.pdd(),
-            W.pmax());
+        hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(), ns);
+    if (W.lds)
+        klaunch(ctx, ku, k_cg_stencil_lds<PS, true>, dim3(ns), dim3(256), (const PT *)pbuf, i1,
+                W.r.as<float4>(), W.x.as<float4>(), G, W.tl, dxf, (const CgState *)st, W.pdd(),
+                W.pmax());
+    else
+        klaunch(ctx, ku, k_cg_update_tm<PS>, dim3(ns), dim3(256), W.r.as<float4>(),
+                W.x.as<float4>(), (const PT *)pbuf, i1, G, W.tm, dxf, (const CgState *)st,
+                W.pdd(), W.pmax());
     if (finalizers)
         hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
-                           W.pmax(), W.nb_tm);
+                           W.pmax(), ns);
     klaunch(ctx, kd, k_cg_dir_tm<PS>, dim3(W.nb_tm), dim3(256), (PT *)pbuf,
             (const float4 *)W.r.as<float4>(), i1, G, W.tm, dxf, (const CgState *)st);
 }
@@ -1295,7 +1613,7 @@ static void enqueue_tm_iter_any(flof_ctx *ctx, int pstore, LevelWork &W, void *p
 static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const float *i1,
                      const LevelGeom &G, float dxf, int pstore, void *pbuf)
 {
-    const std::array<int, 5> key{D.X, D.Y, D.Z, D.T, pstore};
+    const std::array<int, 5> key{D.X, D.Y, D.Z, D.T, tune_variant(ctx)};
     auto it = ctx->tm_tuned.find(key);
     if (it != ctx->tm_tuned.end()) return it->second;
     CgState hs;
@@ -1318,7 +1636,7 @@ static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const float *i
     hipError_t err = hipSuccess;
     for (int bpc : TM_CANDIDATES) {
         W.set_tm(ctx, D, bpc);
-        if (W.nb_tm > W.cap) continue;
+        if (W.nb_tm > W.cap || W.nb_lds > W.cap) continue;
         iterate();  // warm-up
         err = hipEventRecord(e0, ctx->stream);
         iterate();
@@ -1378,7 +1696,8 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
     void *pbuf = pstore == FLOF_PSTORE_F32 ? W.p.p : W.q.p;
     if (ctx->use_tmarch) {
         int bpc = ctx->tm_blocks_per_cu;
-        if (ctx->autotune && n >= (1 << 16)) bpc = tuned_bpc(ctx, D, W, i1, G, dxf, pstore, pbuf);
+        if (ctx->autotune && n >= (1 << 16))
+            bpc = tuned_bpc(ctx, D, W, i1, G, dxf, pstore, pbuf);
         W.set_tm(ctx, D, bpc);
     }
     // with fewer than 3 cells in a dimension every row is an identity row and
@@ -1760,6 +2079,9 @@ static flof_ctx *worker_ctx(flof_ctx *ctx, int w)
         c->blocks_per_cu = ctx->blocks_per_cu;
         c->use_tmarch = ctx->use_tmarch;
         c->tm_blocks_per_cu = ctx->tm_blocks_per_cu;
+        c->lds_stencil = ctx->lds_stencil;
+        c->lds_blocks_per_cu = ctx->lds_blocks_per_cu;
+        c->lds_xcd = ctx->lds_xcd;
         c->autotune = ctx->autotune;
         ctx->workers.push_back(c);
     }

@@ -369,3 +369,43 @@ def test_multiscale_ragged_sizes_bit_exact():
     lv = M.getLastSolveInfo().summary()["levels"][0]
     assert lv["status"] == [3] and lv["iters"] == [0]
     assert not v2.to_numpy().any()
+
+
+@pytest.mark.parametrize("kind", ["lds", "tmarch"])
+def test_stencil_kernels_bit_exact(kind):
+    """Both forms of the PCG stencil passes (LDS tiles of 64x4x4 cells, and
+    the t-marching kernels reading neighbours through the caches) give the
+    oracle's result, on a level whose sizes leave every tile ragged (X > 64,
+    Y, Z not multiples of 4) and with an incoming flow; and with the
+    bfloat16 direction storage the two forms agree bit for bit."""
+    from ofblend_amd import _lib
+    from ofblend_amd.abi import FLOF_STENCIL_LDS, FLOF_STENCIL_TMARCH, FLOF_PSTORE_BF16
+    X, Y, Z, T = 70, 9, 10, 11
+    a = O.drop_sdf(X, Y, Z, T, seed=51)
+    b = O.drop_sdf(X, Y, Z, T, seed=52)
+    vin = (np.random.default_rng(53).standard_normal((T, Z, Y, X, 4)) * 0.5).astype(np.float32)
+    L = _lib.lib()
+    ctx = M.context()
+    s = solver(X, Y, Z, T)
+    ga, gb = real(s, a), real(s, b)
+    say("stencil kernels", kind, (X, Y, Z, T))
+    out = {}
+    try:
+        for k, code in (("lds", FLOF_STENCIL_LDS), ("tmarch", FLOF_STENCIL_TMARCH)):
+            assert L.flof_ctx_set_stencil(ctx, code) == 0
+            for prec in (0, FLOF_PSTORE_BF16):
+                vel = vec4(s, vin)
+                M.opticalFlow4d(vel, ga, gb, wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5,
+                                precision=prec)
+                out[k, prec] = (vel.to_numpy(), M.getLastSolveInfo().summary())
+    finally:
+        L.flof_ctx_set_stencil(ctx, FLOF_STENCIL_LDS)
+    assert L.flof_ctx_set_stencil(ctx, 7) != 0
+    p = default_params(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5)
+    with O.sum_mode(1):
+        ref, it, res, st, rhs = O.optical_flow4d(vin, a, b, p, want_rhs=True)
+    v, info = out[kind, 0]
+    say("iterations", info["levels"][0]["iters"], "oracle", it)
+    assert info["levels"][0]["iters"] == [it]
+    assert np.array_equal(v, ref)
+    assert np.array_equal(out["lds", FLOF_PSTORE_BF16][0], out["tmarch", FLOF_PSTORE_BF16][0])
