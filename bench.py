@@ -38,9 +38,11 @@ B_UPDATE = 16 * 3 + 4 + 16 * 2   # read p,r,x,i1; write r,x
 B_APPLY = 16 + 4                 # read p,i1 (p.Ap only)
 B_DIR = 16 * 2 + 4 + 16          # read r,p,i1; write p
 B_ITER_MIN = 112                 # x,r,p read+written once, g read once
-# kernel names of the PCG launch kinds: the LDS-tiled stencil kernels are the
-# library's default (FLOF_LDS_STENCIL=0 selects the t-marching ones)
-LDS_STENCIL = os.environ.get("FLOF_LDS_STENCIL", "1")[:1] == "1"
+# kernel names of the PCG launch kinds on the finest level: the library runs
+# the LDS-tiled stencil kernels there (64^4, FP32: long t segments of full
+# tiles) and the t-marching ones on the coarser levels; FLOF_LDS_STENCIL=0
+# selects the t-marching kernels everywhere
+LDS_STENCIL = os.environ.get("FLOF_LDS_STENCIL", "auto")[:1] != "0"
 KN_UPDATE = "k_cg_stencil_lds<update>" if LDS_STENCIL else "k_cg_update_tm"
 KN_APPLY = "k_cg_stencil_lds<pq>" if LDS_STENCIL else "k_cg_pq_tm"
 KN_DIR = "k_cg_dir_tm"
@@ -215,6 +217,8 @@ def main():
         "roofline": roofline,
         "kernels_finest_level": kst0,
         "kernels_all_levels": kst,
+        "kernels_all_levels_note": "per launch kind; the coarser levels run the "
+                                   "t-marching p.Ap and update kernels",
         "ms_per_step_with_kernel_events": 1000.0 * float(np.median(t_events)),
     }
 

@@ -150,10 +150,13 @@ int flof_ctx_set_autotune(flof_ctx *ctx, int enable);
 int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T);
 
 /* Kernels of the PCG's two stencil passes, p.(A p) and the update (no
- * reference counterpart; results are identical): FLOF_STENCIL_LDS (the
- * default) tiles 64x4x4 cells in LDS, FLOF_STENCIL_TMARCH reads the
- * neighbours through the caches. */
-enum flof_stencil { FLOF_STENCIL_TMARCH = 0, FLOF_STENCIL_LDS = 1 };
+ * reference counterpart; results are identical): FLOF_STENCIL_LDS tiles
+ * 64x4x4 cells in LDS, FLOF_STENCIL_TMARCH reads the neighbours through the
+ * caches, FLOF_STENCIL_AUTO (the default) takes the LDS tiles for levels
+ * where they are faster (FP32 direction storage, t segments of 16 or more
+ * steps, X filling 90% of the 64-wide tiles) and the t-marching kernels
+ * elsewhere. */
+enum flof_stencil { FLOF_STENCIL_AUTO = 0, FLOF_STENCIL_TMARCH = 1, FLOF_STENCIL_LDS = 2 };
 int flof_ctx_set_stencil(flof_ctx *ctx, int kind);
 
 /* ---- Grid4d storage (source/grid4d.h:117-265, source/grid4d.cpp:60-138) - */

@@ -19,8 +19,9 @@ FLOF_MAX_STEPS = 10
 FLOF_PSTORE_F32 = 0
 FLOF_PSTORE_BF16 = 1
 
-FLOF_STENCIL_TMARCH = 0
-FLOF_STENCIL_LDS = 1
+FLOF_STENCIL_AUTO = 0
+FLOF_STENCIL_TMARCH = 1
+FLOF_STENCIL_LDS = 2
 
 FLOF_KT_CG_APPLY = 0
 FLOF_KT_CG_UPDATE = 1

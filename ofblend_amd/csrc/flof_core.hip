@@ -852,7 +852,7 @@ int flof_ctx_create(flof_ctx **out, int device)
         const char *nt = std::getenv("FLOF_NO_TMARCH");
         ctx->use_tmarch = !(nt && nt[0] == '1');
         const char *ls = std::getenv("FLOF_LDS_STENCIL");
-        if (ls) ctx->lds_stencil = ls[0] == '1';
+        if (ls) ctx->stencil = ls[0] == '1' ? FLOF_STENCIL_LDS : FLOF_STENCIL_TMARCH;
         const char *lx = std::getenv("FLOF_LDS_XCD");
         if (lx) ctx->lds_xcd = lx[0] == '1';
         const char *lb = std::getenv("FLOF_LDS_BPC");
@@ -902,9 +902,9 @@ int flof_ctx_set_autotune(flof_ctx *ctx, int enable)
 
 int flof_ctx_set_stencil(flof_ctx *ctx, int kind)
 {
-    if (!ctx || (kind != FLOF_STENCIL_TMARCH && kind != FLOF_STENCIL_LDS)) return FLOF_ERR_ARG;
-    ctx->lds_stencil = kind == FLOF_STENCIL_LDS;
-    for (flof_ctx *w : ctx->workers) w->lds_stencil = ctx->lds_stencil;
+    if (!ctx || kind < FLOF_STENCIL_AUTO || kind > FLOF_STENCIL_LDS) return FLOF_ERR_ARG;
+    ctx->stencil = kind;
+    for (flof_ctx *w : ctx->workers) w->stencil = kind;
     return FLOF_OK;
 }
 

@@ -87,7 +87,7 @@ struct flof_ctx {
     // LDS-tiled stencil kernels for p.Ap and the update (FLOF_LDS_STENCIL=0
     // selects the t-marching ones), their work items per CU (FLOF_LDS_BPC)
     // and XCD-contiguous work items (FLOF_LDS_XCD=0 disables)
-    bool lds_stencil = true;
+    int stencil = FLOF_STENCIL_AUTO;
     int lds_blocks_per_cu = 2;
     bool lds_xcd = true;
     // device scratch for reductions (partials + state), grown on demand
@@ -108,7 +108,7 @@ struct flof_ctx {
 // the kernel variant part of the autotuner's key of a grid shape
 inline int tune_variant(const flof_ctx *ctx)
 {
-    return ctx->p_store + (ctx->lds_stencil ? 2 : 0);
+    return ctx->p_store + 2 * ctx->stencil;
 }
 
 struct flof_warp_state {

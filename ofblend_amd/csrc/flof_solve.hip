@@ -1530,7 +1530,15 @@ struct LevelWork {
         tl = make_tilemarch(ctx, D, ctx->lds_blocks_per_cu);
         nb_lds = tl.xcd ? ((tl.total + FLOF_NXCD - 1) / FLOF_NXCD) * FLOF_NXCD : tl.total;
         // the LDS kernels index cells (and the t+2 prefetch) in 32 bits
-        lds = ctx->lds_stencil && D.st * (int64_t)(D.T + 2) < (1ll << 31);
+        // the LDS tiles pay off on long t segments of full-width tiles with
+        // FP32 direction storage (measured: 3% faster at 64^4, 35% slower
+        // at 32^4 with 4-step segments, 26% slower with bfloat16 storage);
+        // they index cells (and the t+2 prefetch) in 32 bits
+        const bool fits = D.st * (int64_t)(D.T + 2) < (1ll << 31);
+        const bool pays = ctx->p_store == FLOF_PSTORE_F32 && tl.seglen >= 16 &&
+                          10 * D.X >= 9 * LT_X * tl.ntx;
+        lds = fits && (ctx->stencil == FLOF_STENCIL_LDS ||
+                       (ctx->stencil == FLOF_STENCIL_AUTO && pays));
     }
     bool lds = false;  // p.Ap and the update run as the LDS-tiled kernels
     int nb_stencil() const { return lds ? nb_lds : nb_tm; }
@@ -2079,7 +2087,7 @@ static flof_ctx *worker_ctx(flof_ctx *ctx, int w)
         c->blocks_per_cu = ctx->blocks_per_cu;
         c->use_tmarch = ctx->use_tmarch;
         c->tm_blocks_per_cu = ctx->tm_blocks_per_cu;
-        c->lds_stencil = ctx->lds_stencil;
+        c->stencil = ctx->stencil;
         c->lds_blocks_per_cu = ctx->lds_blocks_per_cu;
         c->lds_xcd = ctx->lds_xcd;
         c->autotune = ctx->autotune;

@@ -379,7 +379,8 @@ def test_stencil_kernels_bit_exact(kind):
     Y, Z not multiples of 4) and with an incoming flow; and with the
     bfloat16 direction storage the two forms agree bit for bit."""
     from ofblend_amd import _lib
-    from ofblend_amd.abi import FLOF_STENCIL_LDS, FLOF_STENCIL_TMARCH, FLOF_PSTORE_BF16
+    from ofblend_amd.abi import (FLOF_PSTORE_BF16, FLOF_STENCIL_AUTO, FLOF_STENCIL_LDS,
+                                 FLOF_STENCIL_TMARCH)
     X, Y, Z, T = 70, 9, 10, 11
     a = O.drop_sdf(X, Y, Z, T, seed=51)
     b = O.drop_sdf(X, Y, Z, T, seed=52)
@@ -399,7 +400,7 @@ def test_stencil_kernels_bit_exact(kind):
                                 precision=prec)
                 out[k, prec] = (vel.to_numpy(), M.getLastSolveInfo().summary())
     finally:
-        L.flof_ctx_set_stencil(ctx, FLOF_STENCIL_LDS)
+        L.flof_ctx_set_stencil(ctx, FLOF_STENCIL_AUTO)
     assert L.flof_ctx_set_stencil(ctx, 7) != 0
     p = default_params(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5)
     with O.sum_mode(1):
