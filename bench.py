@@ -222,6 +222,24 @@ def main():
         "ms_per_step_with_kernel_events": 1000.0 * float(np.median(t_events)),
     }
 
+    # what HBM sustains for the byte mixes of the PCG kernels when a plain
+    # streaming kernel moves them (no stencil work), on the finest level's
+    # cell count: the practical ceiling the kernels are compared with
+    from ofblend_amd.abi import FLOF_MIX_COPY, FLOF_MIX_DIR, FLOF_MIX_UPDATE
+    stream = {}
+    for nm, mix in (("update_mix_84B", FLOF_MIX_UPDATE), ("dir_mix_52B", FLOF_MIX_DIR),
+                    ("copy_32B", FLOF_MIX_COPY)):
+        g = C.c_double()
+        rc = L.flof_stream_bandwidth(ctx, mix, cells0, 20, C.byref(g))
+        if rc == 0:
+            stream[nm] = g.value
+    out["stream_bandwidth_GBps"] = stream
+    same = {KN_UPDATE: "update_mix_84B", KN_DIR: "dir_mix_52B"}.get(dom)
+    if same in stream:
+        roofline["achievable_same_mix"] = {
+            "GBps": stream[same], "frac": achieved / 1e9 / stream[same],
+            "note": "a streaming kernel with the same bytes per cell and no stencil work"}
+
     # host-buffer boundary: the inputs come from and the flow goes to host
     # memory (pageable numpy arrays): copies around one solve
     h0, h1 = i0.to_numpy(), i1.to_numpy()

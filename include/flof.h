@@ -159,6 +159,17 @@ int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T);
 enum flof_stencil { FLOF_STENCIL_AUTO = 0, FLOF_STENCIL_TMARCH = 1, FLOF_STENCIL_LDS = 2 };
 int flof_ctx_set_stencil(flof_ctx *ctx, int kind);
 
+/* Measurement aid (no reference counterpart): the bandwidth a plain
+ * streaming kernel reaches with the byte mix of one PCG kernel and no
+ * stencil work, on `cells` cells of scratch memory taken from the context's
+ * pool: FLOF_MIX_UPDATE reads 3 float4 + 1 float and writes 2 float4 per
+ * cell (84 B), FLOF_MIX_DIR reads 2 float4 + 1 float and writes 1 float4
+ * (52 B), FLOF_MIX_COPY reads and writes 1 float4 (32 B); non-temporal
+ * accesses as in the PCG kernels, best of two launch shapes, each timed
+ * over `reps` launches.  *gbps = bytes moved / time, in GB/s. */
+enum flof_mix { FLOF_MIX_UPDATE = 0, FLOF_MIX_DIR = 1, FLOF_MIX_COPY = 2 };
+int flof_stream_bandwidth(flof_ctx *ctx, int mix, int64_t cells, int reps, double *gbps);
+
 /* ---- Grid4d storage (source/grid4d.h:117-265, source/grid4d.cpp:60-138) - */
 /* Creates a zero-initialised X*Y*Z*T grid (x fastest, then y, z, t). */
 int flof_grid4d_create(flof_ctx *ctx, int X, int Y, int Z, int T, int elem,

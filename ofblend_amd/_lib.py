@@ -47,6 +47,7 @@ SIGNATURES = {
     "flof_ctx_set_autotune": ([_P, _I], _I),
     "flof_ctx_tuned_blocks": ([_P, _I, _I, _I, _I], _I),
     "flof_ctx_set_stencil": ([_P, _I], _I),
+    "flof_stream_bandwidth": ([_P, _I, C.c_int64, _I, _DP], _I),
     "flof_grid4d_create": ([_P, _I, _I, _I, _I, _I, _I, C.POINTER(_P)], _I),
     "flof_grid4d_destroy": ([_P], _I),
     "flof_grid4d_dims": ([_P, C.POINTER(C.c_int), C.POINTER(C.c_int),

@@ -23,6 +23,10 @@ FLOF_STENCIL_AUTO = 0
 FLOF_STENCIL_TMARCH = 1
 FLOF_STENCIL_LDS = 2
 
+FLOF_MIX_UPDATE = 0
+FLOF_MIX_DIR = 1
+FLOF_MIX_COPY = 2
+
 FLOF_KT_CG_APPLY = 0
 FLOF_KT_CG_UPDATE = 1
 FLOF_KT_CG_DIR = 2

@@ -983,6 +983,114 @@ int flof_ctx_timing(flof_ctx *ctx, int kind, int64_t *launches, double *ms_total
     return FLOF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Streaming kernels with the byte mixes of the PCG kernels and no stencil
+// work: the bandwidth HBM sustains for a mix (reads and writes interleave
+// differently from a copy), against which the PCG kernels are compared.
+namespace flof {
+namespace {
+typedef float f4s __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 nt_ld(const float4 *a)
+{
+    const f4s v = __builtin_nontemporal_load((const f4s *)a);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void nt_st(float4 *a, const float4 &v)
+{
+    f4s w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, (f4s *)a);
+}
+// update mix: read x, r, p, g; write x, r (84 B per cell)
+__global__ void __launch_bounds__(256)
+k_stream_update(float4 *__restrict__ x, float4 *__restrict__ r, const float4 *__restrict__ p,
+                const float *__restrict__ g, int64_t n, float a)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const float4 pc = p[c];
+        const float gc = g[c];
+        float4 xc = nt_ld(x + c), rc = nt_ld(r + c);
+        xc = make_float4(xc.x + a * pc.x, xc.y + a * pc.y, xc.z + a * pc.z, xc.w + a * pc.w);
+        rc = make_float4(rc.x - gc * pc.x, rc.y - gc * pc.y, rc.z - gc * pc.z, rc.w - gc * pc.w);
+        nt_st(x + c, xc);
+        nt_st(r + c, rc);
+    }
+}
+// direction mix: read r, p, g; write p (52 B per cell)
+__global__ void __launch_bounds__(256)
+k_stream_dir(float4 *__restrict__ p, const float4 *__restrict__ r, const float *__restrict__ g,
+             int64_t n, float b)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const float4 rc = nt_ld(r + c), pc = nt_ld(p + c);
+        const float gc = g[c];
+        nt_st(p + c, make_float4(rc.x * gc + b * pc.x, rc.y * gc + b * pc.y,
+                                 rc.z * gc + b * pc.z, rc.w * gc + b * pc.w));
+    }
+}
+// copy: read a, write b (32 B per cell)
+__global__ void __launch_bounds__(256)
+k_stream_copy(float4 *__restrict__ b, const float4 *__restrict__ a, int64_t n)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x)
+        nt_st(b + c, nt_ld(a + c));
+}
+}  // namespace
+}  // namespace flof
+
+int flof_stream_bandwidth(flof_ctx *ctx, int mix, int64_t cells, int reps, double *gbps)
+{
+    API_TRY
+    FLOF_REQUIRE(ctx && mix >= 0 && mix <= 2 && cells > 0 && reps > 0 && gbps, FLOF_ERR_ARG,
+                 "flof_stream_bandwidth: bad argument");
+    const size_t v4 = (size_t)cells * 16;
+    Tmp a(ctx, v4), b(ctx, v4), c(ctx, v4), g(ctx, (size_t)cells * 4);
+    FLOF_HIP(hipMemsetAsync(a.p, 0, v4, ctx->stream));
+    FLOF_HIP(hipMemsetAsync(b.p, 0, v4, ctx->stream));
+    FLOF_HIP(hipMemsetAsync(c.p, 0, v4, ctx->stream));
+    FLOF_HIP(hipMemsetAsync(g.p, 0, (size_t)cells * 4, ctx->stream));
+    const double bytes = (mix == 0 ? 84.0 : mix == 1 ? 52.0 : 32.0) * (double)cells;
+    hipEvent_t e0, e1;
+    FLOF_HIP(hipEventCreate(&e0));
+    FLOF_HIP(hipEventCreate(&e1));
+    double best = 0.0;
+    hipError_t err = hipSuccess;
+    for (int bpc : {4, 8}) {
+        const int grid = ctx->num_cu * bpc;
+        auto launch = [&]() {
+            if (mix == 0)
+                hipLaunchKernelGGL(k_stream_update, dim3(grid), dim3(256), 0, ctx->stream,
+                                   a.as<float4>(), b.as<float4>(), (const float4 *)c.as<float4>(),
+                                   (const float *)g.as<float>(), cells, 0.5f);
+            else if (mix == 1)
+                hipLaunchKernelGGL(k_stream_dir, dim3(grid), dim3(256), 0, ctx->stream,
+                                   a.as<float4>(), (const float4 *)b.as<float4>(),
+                                   (const float *)g.as<float>(), cells, 0.5f);
+            else
+                hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream,
+                                   a.as<float4>(), (const float4 *)b.as<float4>(), cells);
+        };
+        launch();  // warm-up
+        launch();
+        err = hipEventRecord(e0, ctx->stream);
+        for (int k = 0; k < reps && err == hipSuccess; ++k) launch();
+        if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        if (err != hipSuccess) break;
+        if (ms > 0.f) best = std::max(best, bytes * reps / (ms * 1e-3) / 1e9);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    FLOF_HIP(err);
+    FLOF_HIP(hipGetLastError());
+    *gbps = best;
+    API_CATCH(ctx)
+}
+
 int flof_grid4d_create(flof_ctx *ctx, int X, int Y, int Z, int T, int elem, int dtype,
                        flof_grid4d **out)
 {
