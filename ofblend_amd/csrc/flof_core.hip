@@ -852,7 +852,8 @@ int flof_ctx_create(flof_ctx **out, int device)
         const char *nt = std::getenv("FLOF_NO_TMARCH");
         ctx->use_tmarch = !(nt && nt[0] == '1');
         const char *ls = std::getenv("FLOF_LDS_STENCIL");
-        if (ls) ctx->stencil = ls[0] == '1' ? FLOF_STENCIL_LDS : FLOF_STENCIL_TMARCH;
+        if (ls && ls[0] == '0') ctx->stencil = FLOF_STENCIL_TMARCH;
+        if (ls && ls[0] == '1') ctx->stencil = FLOF_STENCIL_LDS;
         const char *lx = std::getenv("FLOF_LDS_XCD");
         if (lx) ctx->lds_xcd = lx[0] == '1';
         const char *lb = std::getenv("FLOF_LDS_BPC");

@@ -84,9 +84,10 @@ struct flof_ctx {
     // CU (FLOF_TM_BPC)
     bool use_tmarch = true;
     int tm_blocks_per_cu = 4;
-    // LDS-tiled stencil kernels for p.Ap and the update (FLOF_LDS_STENCIL=0
-    // selects the t-marching ones), their work items per CU (FLOF_LDS_BPC)
-    // and XCD-contiguous work items (FLOF_LDS_XCD=0 disables)
+    // kernels of the two PCG stencil passes: FLOF_STENCIL_AUTO / _TMARCH /
+    // _LDS (FLOF_LDS_STENCIL=0 or 1 forces the t-marching or the LDS-tiled
+    // ones), work items per CU of the LDS-tiled ones (FLOF_LDS_BPC) and
+    // XCD-contiguous work items (FLOF_LDS_XCD=0 disables)
     int stencil = FLOF_STENCIL_AUTO;
     int lds_blocks_per_cu = 2;
     bool lds_xcd = true;
