@@ -410,3 +410,31 @@ def test_stencil_kernels_bit_exact(kind):
     assert info["levels"][0]["iters"] == [it]
     assert np.array_equal(v, ref)
     assert np.array_equal(out["lds", FLOF_PSTORE_BF16][0], out["tmarch", FLOF_PSTORE_BF16][0])
+
+
+def test_stencil_kernels_degenerate_shapes():
+    """The LDS-tiled kernels (forced) on shapes with at most a few cells in
+    some dimension, tiles sticking out on every side and one-step t
+    segments: the same flow and iterations as the t-marching kernels."""
+    from ofblend_amd import _lib
+    from ofblend_amd.abi import FLOF_STENCIL_AUTO, FLOF_STENCIL_LDS, FLOF_STENCIL_TMARCH
+    L = _lib.lib()
+    ctx = M.context()
+    try:
+        for n, (X, Y, Z, T) in enumerate([(3, 2, 1, 5), (65, 5, 5, 2), (8, 8, 8, 1),
+                                          (130, 3, 6, 4), (5, 9, 3, 17)]):
+            say("degenerate shape", (X, Y, Z, T))
+            a = O.drop_sdf(X, Y, Z, T, seed=70 + 2 * n)
+            b = O.drop_sdf(X, Y, Z, T, seed=71 + 2 * n)
+            s = solver(X, Y, Z, T)
+            ga, gb = real(s, a), real(s, b)
+            res = []
+            for code in (FLOF_STENCIL_TMARCH, FLOF_STENCIL_LDS):
+                assert L.flof_ctx_set_stencil(ctx, code) == 0
+                vel = s.create(M.Grid4Vec4)
+                M.opticalFlow4d(vel, ga, gb, wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5)
+                res.append((vel.to_numpy(), M.getLastSolveInfo().summary()["levels"][0]["iters"]))
+            assert res[0][1] == res[1][1]
+            assert np.array_equal(res[0][0], res[1][0])
+    finally:
+        L.flof_ctx_set_stencil(ctx, FLOF_STENCIL_AUTO)
