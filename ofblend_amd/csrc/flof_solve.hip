@@ -15,11 +15,15 @@
 // applied from g on the fly with the identical FP32 operation order, so
 // q = A p is bit-identical to the reference's applyMat.
 //
-// PCG per iteration: three t-marching kernels (k_cg_pq_tm, k_cg_update_tm,
-// k_cg_dir_tm, 156 B per cell) with two 1-block FP64 finalizers between them
-// (k_cg_alpha: alpha = sigma / p.q; k_cg_beta: stopping test, beta).  The
-// streaming variant with a q array (k_cg_apply, k_cg_update, k_cg_dir,
-// 208 B per cell) is kept for comparison (FLOF_NO_TMARCH=1).
+// PCG per iteration: three stencil kernels (p.q, the x / r update with z.r
+// and max r, the direction update; 156 B per cell) with two 1-block FP64
+// finalizers between them (k_cg_alpha: alpha = sigma / p.q; k_cg_beta:
+// stopping test, beta).  p.q and the update run as LDS-tiled kernels
+// (k_cg_stencil_lds) on levels with long t segments and as t-marching
+// kernels (k_cg_pq_tm, k_cg_update_tm) elsewhere; the direction update is
+// k_cg_dir_tm.  The streaming variant with a q array (k_cg_apply,
+// k_cg_update, k_cg_dir, 208 B per cell) is kept for comparison
+// (FLOF_NO_TMARCH=1).
 // The host enqueues iterations in chunks and reads the device state once per
 // chunk; kernels of iterations past convergence return immediately.
 
@@ -37,15 +41,15 @@
 #ifndef FLOF_TM_UNROLL
 #define FLOF_TM_UNROLL 1
 #endif
-// non-temporal access to data a kernel touches once (1: r and x of the
-// update kernel; 2: also p's own load and store in the direction kernel).
-// Measured at 64^4: 626 -> 565 us per PCG iteration.
 // 1: t-march work items dealt XCD-contiguously (blockIdx % 8 = XCD);
 // 0: in launch order.  Launch order measured 2.5% faster per PCG iteration
 // at 64^4 (its z-neighbour chunks, 16 blocks apart, share an XCD).
 #ifndef FLOF_TM_XCD_MAP
 #define FLOF_TM_XCD_MAP 0
 #endif
+// non-temporal access to data a kernel touches once (1: r and x of the
+// update kernel; 2: also p's own load and store in the direction kernel).
+// Measured at 64^4: 626 -> 565 us per PCG iteration.
 #ifndef FLOF_STREAM_NT
 #define FLOF_STREAM_NT 2
 #endif
@@ -1568,7 +1572,7 @@ struct LevelWork {
     float *pmax() const { return (float *)(part.as<DD>() + 3 * cap); }
 };
 
-// one fused PCG iteration (three t-marching kernels and the two finalizers);
+// one PCG iteration (the three stencil kernels and the two finalizers);
 // pbuf holds p in the storage of PS; ka/ku/kd: timer kinds or -1
 template <class PS>
 static void enqueue_tm_iter(flof_ctx *ctx, LevelWork &W, void *pbuf, const float *i1,
@@ -1612,7 +1616,7 @@ static void enqueue_tm_iter_any(flof_ctx *ctx, int pstore, LevelWork &W, void *p
 }
 
 // Autotuner of the t-march decomposition: on the first solve of a grid
-// shape, times one PCG iteration (the three t-marching kernels, twice) for
+// shape, times one PCG iteration (the three stencil kernels, twice) for
 // each candidate number of workgroups per CU on the level's own (not yet
 // initialised) workspace, with a scratch solver state that lets every kernel
 // run, and keeps the fastest per shape for the context's lifetime.  The
