@@ -74,6 +74,10 @@ def main():
     ap.add_argument("--pmc-workload", action="store_true",
                     help="short workload for rocprofv3 --pmc runs: a 256 MiB "
                          "grid copy (calibration) and 3 level-0 PCG iterations")
+    ap.add_argument("--warp-workload", action="store_true",
+                    help="short workload for rocprofv3 --pmc runs of the frame "
+                         "warp: a 64^4 solve capped at 50 iterations per level, "
+                         "then the bench's 64-frame 128^3 warp; prints its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; every box here has one "
@@ -102,6 +106,11 @@ def main():
     M.dropSdf4d(i0, seed=1)
     M.dropSdf4d(i1, seed=2)
     vel = s.create(M.Grid4Vec4)
+    if args.warp_workload:
+        M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, cgMaxIter=50, **PARAMS)
+        M.sync()
+        print(json.dumps(frame_warp(M, L, ctx, s, vel, FLOF_KT_WARP)), flush=True)
+        return
     if args.pmc_workload:
         # copy of a 64^4 Vec4 grid: 256 MiB read + 256 MiB written
         other = s.create(M.Grid4Vec4)
