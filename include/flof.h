@@ -159,6 +159,35 @@ int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T);
 enum flof_stencil { FLOF_STENCIL_AUTO = 0, FLOF_STENCIL_TMARCH = 1, FLOF_STENCIL_LDS = 2 };
 int flof_ctx_set_stencil(flof_ctx *ctx, int kind);
 
+/* ---- Output post-processing of warped 3D frames (scenes/flof.py:737-782,
+ * scenes/ofHelpers.py:108-115).  A frame is a float[Z][Y][X] device buffer;
+ * Z == 1 is a 2D grid. */
+
+/* Scalar frame operations the union blends and the temporal merge are made
+ * of: COPY a = b (Grid::copyFrom, source/grid.cpp:198-207), SET_CONST a = s
+ * (grid.cpp:230-233), MULT_CONST a *= s (gridMultScalar, source/grid.h:735),
+ * ADD_SCALED a += s * b (gridScaledAdd, grid.h:740-743), JOIN a = min(a, b)
+ * (the level-set union KnJoin, source/levelset.cpp:114-118).  n cells. */
+enum flof_frame_op_kind {
+    FLOF_FRAME_COPY = 0,
+    FLOF_FRAME_SET_CONST = 1,
+    FLOF_FRAME_MULT_CONST = 2,
+    FLOF_FRAME_ADD_SCALED = 3,
+    FLOF_FRAME_JOIN = 4
+};
+int flof_frame_op(flof_ctx *ctx, int op, float *a, const float *b, int64_t n, float s);
+
+/* simpleBlurSpecial (source/test.cpp:94-127): iter passes of the 6- (2D: 4-)
+ * neighbour mean over the cells >= thresh inside the bord band, the outer
+ * layer of the grid swapped with a zeroed temporary like the reference's. */
+int flof_simple_blur_special(flof_ctx *ctx, float *a, int X, int Y, int Z, int iter,
+                             float thresh, int bord);
+
+/* Grid::setBound (source/grid.cpp:462-473): value into the cells within
+ * width of the border (z only for 3D grids). */
+int flof_frame_set_bound(flof_ctx *ctx, float *a, int X, int Y, int Z, float value,
+                         int width);
+
 /* Measurement aid (no reference counterpart): the bandwidth a plain
  * streaming kernel reaches with the byte mix of one PCG kernel and no
  * stencil work, on `cells` cells of scratch memory taken from the context's

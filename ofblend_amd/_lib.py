@@ -48,6 +48,9 @@ SIGNATURES = {
     "flof_ctx_tuned_blocks": ([_P, _I, _I, _I, _I], _I),
     "flof_ctx_set_stencil": ([_P, _I], _I),
     "flof_stream_bandwidth": ([_P, _I, C.c_int64, _I, _DP], _I),
+    "flof_frame_op": ([_P, _I, _P, _P, C.c_int64, _F], _I),
+    "flof_simple_blur_special": ([_P, _P, _I, _I, _I, _I, _F, _I], _I),
+    "flof_frame_set_bound": ([_P, _P, _I, _I, _I, _F, _I], _I),
     "flof_grid4d_create": ([_P, _I, _I, _I, _I, _I, _I, C.POINTER(_P)], _I),
     "flof_grid4d_destroy": ([_P], _I),
     "flof_grid4d_dims": ([_P, C.POINTER(C.c_int), C.POINTER(C.c_int),

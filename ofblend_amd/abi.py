@@ -27,6 +27,12 @@ FLOF_MIX_UPDATE = 0
 FLOF_MIX_DIR = 1
 FLOF_MIX_COPY = 2
 
+FLOF_FRAME_COPY = 0
+FLOF_FRAME_SET_CONST = 1
+FLOF_FRAME_MULT_CONST = 2
+FLOF_FRAME_ADD_SCALED = 3
+FLOF_FRAME_JOIN = 4
+
 FLOF_KT_CG_APPLY = 0
 FLOF_KT_CG_UPDATE = 1
 FLOF_KT_CG_DIR = 2

@@ -19,8 +19,9 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .abi import (FLOF_F32, FLOF_INT, FLOF_REAL, FLOF_VEC4, OfParams,
-                  SolveInfo, default_params)
+from .abi import (FLOF_F32, FLOF_FRAME_ADD_SCALED, FLOF_FRAME_COPY, FLOF_FRAME_JOIN,
+                  FLOF_FRAME_MULT_CONST, FLOF_FRAME_SET_CONST, FLOF_INT, FLOF_REAL,
+                  FLOF_VEC4, OfParams, SolveInfo, default_params)
 from ._lib import FlofError, check, floats
 
 
@@ -410,6 +411,43 @@ class DeviceArray:
                                            self.ptr, self.nbytes), self.ctx)
         return a
 
+    # Grid<Real> / LevelsetGrid methods of a (Z, Y, X) frame (Z == 1: 2D)
+    def _op(self, op, other, s):
+        if other is not None and other.shape != self.shape:
+            raise FlofError(-1, "frame shapes differ: %s vs %s" % (self.shape, other.shape))
+        n = int(np.prod(self.shape))
+        check(_lib.lib().flof_frame_op(self.ctx, op, self.ptr,
+                                       other.ptr if other is not None else None, n,
+                                       float(s)), self.ctx)
+        return self
+
+    def copyFrom(self, other):
+        """source/grid.cpp:198-207"""
+        return self._op(FLOF_FRAME_COPY, other, 0.)
+
+    def setConst(self, value):
+        """source/grid.cpp:230-233"""
+        return self._op(FLOF_FRAME_SET_CONST, None, value)
+
+    def multConst(self, s):
+        """gridMultScalar, source/grid.h:735"""
+        return self._op(FLOF_FRAME_MULT_CONST, None, s)
+
+    def addScaled(self, other, factor):
+        """gridScaledAdd, source/grid.h:740-743"""
+        return self._op(FLOF_FRAME_ADD_SCALED, other, factor)
+
+    def join(self, other):
+        """LevelsetGrid::join, source/levelset.cpp:114-118"""
+        return self._op(FLOF_FRAME_JOIN, other, 0.)
+
+    def setBound(self, value=0., boundaryWidth=1):
+        """Grid::setBound, source/grid.cpp:462-473"""
+        z, y, x = self.shape
+        check(_lib.lib().flof_frame_set_bound(self.ctx, self.ptr, x, y, z, float(value),
+                                              int(boundaryWidth)), self.ctx)
+        return self
+
     def __del__(self):
         if getattr(self, "ptr", None) is not None and self.ptr.value:
             try:
@@ -625,6 +663,94 @@ def dropFrames3d(frames, X, Y, Z, T, scale, seed, nspheres=1, rmin=0.15,
     check(_lib.lib().flof_drop_frames3d(frames.ctx, frames.ptr, X, Y, Z, T,
                                         scale, seed, nspheres, rmin, rmax,
                                         clampCells), frames.ctx)
+
+
+def simpleBlurSpecial(a, iter=1, thresh=0., bord=0):
+    """source/test.cpp:121-127 on a (Z, Y, X) DeviceArray, in place."""
+    z, y, x = a.shape
+    check(_lib.lib().flof_simple_blur_special(a.ctx, a.ptr, x, y, z, int(iter),
+                                              float(thresh), int(bord)), a.ctx)
+
+
+# --- blend weights of the union blends (scenes/flof.py:576-585,
+# scenes/ofHelpers.py:128-245).  vec3 components are single precision as in
+# the reference's Python binding; the rest is Python (double) arithmetic.
+
+def blendW(t, l):
+    """flof.py:576-585: weights (w0, wu, w1) of the two-way union blend."""
+    w0 = (1.0 + l - 2.0 * t) / (1.0 + l)
+    w1 = (-1.0 + l + 2.0 * t) / (1.0 + l)
+    if w0 < 0.:
+        w0 = 0.
+    if w1 < 0.:
+        w1 = 0.
+    if w0 > 1.:
+        w0 = 1.
+    if w1 > 1.:
+        w1 = 1.
+    wu = 1. - w0 - w1
+    return (w0, wu, w1)
+
+
+def _v3(*c):
+    return tuple(float(np.float32(x)) for x in c)
+
+
+def _v3sub(a, b):
+    return tuple(float(np.float32(np.float32(x) - np.float32(y))) for x, y in zip(a, b))
+
+
+def world2uvw(p, r1, r2, r3):
+    """ofHelpers.py:128-167: barycentric (u, v, w) of p (x, y) in the
+    triangle r1, r2, r3; raises for a degenerate triangle or a point
+    outside it (the reference exits)."""
+    r13 = _v3sub(r1, r3)
+    r23 = _v3sub(r2, r3)
+    ma, mc = r13[0], r13[1]
+    mb, md = r23[0], r23[1]
+    if abs(ma * md - mb * mc) < 1e-05:
+        raise FlofError(-1, "world2uvw: invalid coords")
+    mdet = 1. / (ma * md - mb * mc)
+    ina = mdet * md
+    inb = mdet * -mb
+    inc = mdet * -mc
+    ind = mdet * ma
+    pr3 = _v3sub(p, r3)
+    triu = ina * pr3[0] + inb * pr3[1]
+    triv = inc * pr3[0] + ind * pr3[1]
+    triw = 1. - triu - triv
+    eps = 1e-03
+    if (triu < -eps or triv < -eps or triw < -eps or triu > 1. + eps or triv > 1. + eps
+            or triw > 1. + eps):
+        raise FlofError(-1, "world2uvw: invalid UVW")
+    # the reference's clamp01(triu), clamp01(triv) discard their results
+    triw = 1. - triu - triv
+    return _v3(triu, triv, triw)
+
+
+def weights2dim(uvw):
+    """ofHelpers.py:174-245: weights (w0, w1, w2, w_uv, w_vw, w_uw, w_uvw) of
+    the three-way union blend at barycentric position uvw."""
+    u, v, w = _v3(*uvw)
+    w0 = w1 = w2 = w_uv = w_vw = w_uw = w_uvw = 0.
+    p = _v3(u, v, w)
+    third = 1. / 3.
+    if u >= 0.5:
+        w0, w_uv, w_uw = world2uvw(p, _v3(1., 0., 0.), _v3(.5, .5, 0.), _v3(.5, 0., .5))
+    elif v >= 0.5:
+        w1, w_uv, w_vw = world2uvw(p, _v3(0., 1., 0.), _v3(.5, .5, 0.), _v3(0., .5, .5))
+    elif w >= 0.5:
+        w2, w_uw, w_vw = world2uvw(p, _v3(0., 0., 1.), _v3(.5, 0., .5), _v3(0., .5, .5))
+    elif u >= v and u >= w:
+        w_uv, w_uw, w_uvw = world2uvw(p, _v3(.5, .5, 0.), _v3(.5, 0., .5),
+                                      _v3(third, third, third))
+    elif v >= u and v >= w:
+        w_uv, w_vw, w_uvw = world2uvw(p, _v3(.5, .5, 0.), _v3(0., .5, .5),
+                                      _v3(third, third, third))
+    elif w >= u and w >= v:
+        w_uw, w_vw, w_uvw = world2uvw(p, _v3(.5, 0., .5), _v3(0., .5, .5),
+                                      _v3(third, third, third))
+    return (w0, w1, w2, w_uv, w_vw, w_uw, w_uvw)
 
 
 def sync():
