@@ -461,8 +461,8 @@ def warm_start(M, s, vel, i0, i1, v_plain, iters_plain, ms_plain):
 
 
 def frame_warp(M, L, ctx, s, vel, kind):
-    """Config 2's second part: 64 analytic 3D frames at 128^3 placed into a
-    128^3 x 64 level set and warped by the solved deformation (alpha 1,
+    """The frame warp workload: 64 analytic 3D frames at 128^3 placed into a
+    128^3 x 64 level set and warped by the solved 64^4 deformation (alpha 1,
     defoFactor (2,2,2,1))."""
     F = 2
     fr = M.DeviceArray((N, F * N, F * N, F * N))

@@ -1,13 +1,15 @@
 """Writes tests/golden/digests.json: SHA-256 digests and convergence records
-of the CPU oracle's FlOF solves at the benchmark configurations, for the GPU
-parity tests at sizes whose oracle run takes minutes (config 1 32^4:
-~45 s; config 2 64^4: ~25 min on 6 cores).
+of the CPU oracle's FlOF solves of the seeded drop pairs (seeds 1, 2), for
+the GPU parity tests at sizes whose oracle run takes minutes: drop32_l3
+(32^4, 3 levels, minGridSize 8: ~45 s) and drop64_l3_cap1000 (64^4, 3
+levels, minGridSize 16, the reference's 1000-iteration cap: ~25 min on 6
+cores).
 
 The oracle runs with correctly rounded FP64 sums (sum_mode 1), the summation
 the GPU library uses, so a matching digest means a bit-identical flow field.
 The serial-sum (reference order) oracle's distance to it is recorded too.
 
-    python tests/golden/make_digests.py [c1] [c2]
+    python tests/golden/make_digests.py [drop32_l3] [drop64_l3_cap1000]
 """
 import hashlib
 import json
@@ -23,8 +25,8 @@ from oracle import oracle as O  # noqa: E402
 from ofblend_amd.abi import default_params  # noqa: E402
 
 CASES = {
-    "c1": dict(N=32, minGridSize=8, cgMaxIter=1000),
-    "c2": dict(N=64, minGridSize=16, cgMaxIter=1000),
+    "drop32_l3": dict(N=32, minGridSize=8, cgMaxIter=1000),
+    "drop64_l3_cap1000": dict(N=64, minGridSize=16, cgMaxIter=1000),
 }
 
 
@@ -55,7 +57,7 @@ def run(name):
 
 
 def main():
-    names = [a for a in sys.argv[1:] if a in CASES] or ["c1"]
+    names = [a for a in sys.argv[1:] if a in CASES] or ["drop32_l3"]
     path = os.path.join(HERE, "digests.json")
     data = json.load(open(path)) if os.path.exists(path) else {}
     for n in names:

@@ -211,9 +211,9 @@ def _digests():
     return json.load(open(path)) if os.path.exists(path) else {}
 
 
-@pytest.mark.parametrize("case", ["c1", "c2"])
+@pytest.mark.parametrize("case", ["drop32_l3", "drop64_l3_cap1000"])
 def test_multiscale_golden_digest(case):
-    """Configs 1 (32^4, 3 levels) and 2 (64^4, 3 levels), P_plain: the GPU
+    """The 32^4 and 64^4 drop-pair solves (3 levels), P_plain: the GPU
     flow field is bit-identical to the oracle's (correctly rounded sums),
     whose SHA-256 tests/golden/make_digests.py recorded; the digest file also
     records the serial-sum oracle's distance to it."""
