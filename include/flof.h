@@ -155,8 +155,15 @@ int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T);
  * caches, FLOF_STENCIL_AUTO (the default) takes the LDS tiles for levels
  * where they are faster (FP32 direction storage, t segments of 16 or more
  * steps, X filling 90% of the 64-wide tiles) and the t-marching kernels
- * elsewhere. */
-enum flof_stencil { FLOF_STENCIL_AUTO = 0, FLOF_STENCIL_TMARCH = 1, FLOF_STENCIL_LDS = 2 };
+ * elsewhere; FLOF_STENCIL_QARRAY is the first GPU form (three streaming
+ * kernels that store q = A p and g, 208 B per cell and iteration, FP32
+ * direction storage only), kept for comparison. */
+enum flof_stencil {
+    FLOF_STENCIL_AUTO = 0,
+    FLOF_STENCIL_TMARCH = 1,
+    FLOF_STENCIL_LDS = 2,
+    FLOF_STENCIL_QARRAY = 3
+};
 int flof_ctx_set_stencil(flof_ctx *ctx, int kind);
 
 /* ---- Output post-processing of warped 3D frames (scenes/flof.py:737-782,

@@ -22,6 +22,7 @@ FLOF_PSTORE_BF16 = 1
 FLOF_STENCIL_AUTO = 0
 FLOF_STENCIL_TMARCH = 1
 FLOF_STENCIL_LDS = 2
+FLOF_STENCIL_QARRAY = 3
 
 FLOF_MIX_UPDATE = 0
 FLOF_MIX_DIR = 1

@@ -903,9 +903,14 @@ int flof_ctx_set_autotune(flof_ctx *ctx, int enable)
 
 int flof_ctx_set_stencil(flof_ctx *ctx, int kind)
 {
-    if (!ctx || kind < FLOF_STENCIL_AUTO || kind > FLOF_STENCIL_LDS) return FLOF_ERR_ARG;
-    ctx->stencil = kind;
-    for (flof_ctx *w : ctx->workers) w->stencil = kind;
+    if (!ctx || kind < FLOF_STENCIL_AUTO || kind > FLOF_STENCIL_QARRAY) return FLOF_ERR_ARG;
+    const bool tm = kind != FLOF_STENCIL_QARRAY;
+    ctx->stencil = tm ? kind : FLOF_STENCIL_AUTO;
+    ctx->use_tmarch = tm;
+    for (flof_ctx *w : ctx->workers) {
+        w->stencil = ctx->stencil;
+        w->use_tmarch = tm;
+    }
     return FLOF_OK;
 }
 

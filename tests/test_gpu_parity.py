@@ -371,16 +371,17 @@ def test_multiscale_ragged_sizes_bit_exact():
     assert not v2.to_numpy().any()
 
 
-@pytest.mark.parametrize("kind", ["lds", "tmarch"])
+@pytest.mark.parametrize("kind", ["lds", "tmarch", "qarray"])
 def test_stencil_kernels_bit_exact(kind):
-    """Both forms of the PCG stencil passes (LDS tiles of 64x4x4 cells, and
-    the t-marching kernels reading neighbours through the caches) give the
-    oracle's result, on a level whose sizes leave every tile ragged (X > 64,
-    Y, Z not multiples of 4) and with an incoming flow; and with the
-    bfloat16 direction storage the two forms agree bit for bit."""
+    """Every form of the PCG stencil passes (LDS tiles of 64x4x4 cells, the
+    t-marching kernels reading neighbours through the caches, the streaming
+    kernels with a stored q) gives the oracle's result, on a level whose
+    sizes leave every tile ragged (X > 64, Y, Z not multiples of 4) and with
+    an incoming flow; and with the bfloat16 direction storage the two fused
+    forms agree bit for bit."""
     from ofblend_amd import _lib
     from ofblend_amd.abi import (FLOF_PSTORE_BF16, FLOF_STENCIL_AUTO, FLOF_STENCIL_LDS,
-                                 FLOF_STENCIL_TMARCH)
+                                 FLOF_STENCIL_QARRAY, FLOF_STENCIL_TMARCH)
     X, Y, Z, T = 70, 9, 10, 11
     a = O.drop_sdf(X, Y, Z, T, seed=51)
     b = O.drop_sdf(X, Y, Z, T, seed=52)
@@ -392,9 +393,10 @@ def test_stencil_kernels_bit_exact(kind):
     say("stencil kernels", kind, (X, Y, Z, T))
     out = {}
     try:
-        for k, code in (("lds", FLOF_STENCIL_LDS), ("tmarch", FLOF_STENCIL_TMARCH)):
+        for k, code in (("lds", FLOF_STENCIL_LDS), ("tmarch", FLOF_STENCIL_TMARCH),
+                        ("qarray", FLOF_STENCIL_QARRAY)):
             assert L.flof_ctx_set_stencil(ctx, code) == 0
-            for prec in (0, FLOF_PSTORE_BF16):
+            for prec in ((0,) if k == "qarray" else (0, FLOF_PSTORE_BF16)):
                 vel = vec4(s, vin)
                 M.opticalFlow4d(vel, ga, gb, wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5,
                                 precision=prec)
@@ -413,11 +415,13 @@ def test_stencil_kernels_bit_exact(kind):
 
 
 def test_stencil_kernels_degenerate_shapes():
-    """The LDS-tiled kernels (forced) on shapes with at most a few cells in
-    some dimension, tiles sticking out on every side and one-step t
-    segments: the same flow and iterations as the t-marching kernels."""
+    """The LDS-tiled kernels (forced) and the streaming q-array kernels on
+    shapes with at most a few cells in some dimension, tiles sticking out on
+    every side and one-step t segments: the same flow and iterations as the
+    t-marching kernels."""
     from ofblend_amd import _lib
-    from ofblend_amd.abi import FLOF_STENCIL_AUTO, FLOF_STENCIL_LDS, FLOF_STENCIL_TMARCH
+    from ofblend_amd.abi import (FLOF_STENCIL_AUTO, FLOF_STENCIL_LDS, FLOF_STENCIL_QARRAY,
+                                 FLOF_STENCIL_TMARCH)
     L = _lib.lib()
     ctx = M.context()
     try:
@@ -429,12 +433,13 @@ def test_stencil_kernels_degenerate_shapes():
             s = solver(X, Y, Z, T)
             ga, gb = real(s, a), real(s, b)
             res = []
-            for code in (FLOF_STENCIL_TMARCH, FLOF_STENCIL_LDS):
+            for code in (FLOF_STENCIL_TMARCH, FLOF_STENCIL_LDS, FLOF_STENCIL_QARRAY):
                 assert L.flof_ctx_set_stencil(ctx, code) == 0
                 vel = s.create(M.Grid4Vec4)
                 M.opticalFlow4d(vel, ga, gb, wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5)
                 res.append((vel.to_numpy(), M.getLastSolveInfo().summary()["levels"][0]["iters"]))
-            assert res[0][1] == res[1][1]
-            assert np.array_equal(res[0][0], res[1][0])
+            for r in res[1:]:
+                assert r[1] == res[0][1]
+                assert np.array_equal(r[0], res[0][0])
     finally:
         L.flof_ctx_set_stencil(ctx, FLOF_STENCIL_AUTO)
