@@ -240,7 +240,8 @@ int flof_grid4d_get_max_abs(flof_grid4d *g, float *out);
 /* setBound / setBoundNeumann (grid4d.cpp:355-412). */
 int flof_grid4d_set_bound(flof_grid4d *g, const float *value, int width);
 int flof_grid4d_set_bound_neumann(flof_grid4d *g, int width);
-/* grid4dMaxDiff / grid4dMaxDiffVec4 (grid4d.cpp:419-464). */
+/* grid4dMaxDiff / grid4dMaxDiffVec4 (grid4d.cpp:419-464); *out holds the
+ * reference's Real (FP32) result. */
 int flof_grid4d_max_diff(const flof_grid4d *a, const flof_grid4d *b,
                          double *out);
 /* setRegion4d / setRegion4dVec4 (grid4d.cpp:466-483). */

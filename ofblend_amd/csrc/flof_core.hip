@@ -636,8 +636,8 @@ __global__ void k_set_region(T *g, Dims4 D, Coord4 C, float4 s, float4 e, T valu
 template <class T>
 __global__ void k_maxdiff(const T *a, const T *b, int64_t n, double *part)
 {
-    // per cell: |a-b| (REAL) or the sum of the 4 component differences (VEC4),
-    // formed in double as the reference does (grid4d.cpp:419-464)
+    // per cell: |a-b| of the FP32 difference (REAL) or the sum of the 4
+    // component differences formed in double (VEC4), as grid4d.cpp:419-464
     double mx = 0.;
     FOR_XCD(i, n)
     {
@@ -647,7 +647,7 @@ __global__ void k_maxdiff(const T *a, const T *b, int64_t n, double *part)
             d = fabs((double)x.x - (double)y.x) + fabs((double)x.y - (double)y.y) +
                 fabs((double)x.z - (double)y.z) + fabs((double)x.w - (double)y.w);
         } else {
-            d = fabs((double)((const float *)a)[i] - (double)((const float *)b)[i]);
+            d = (double)fabsf(((const float *)a)[i] - ((const float *)b)[i]);
         }
         mx = fmax(mx, d);
     }
@@ -665,7 +665,7 @@ __global__ void k_maxdiff_fin(const double *part, int nb, double *result)
     if (threadIdx.x == 0) {
         double m = 0.;
         for (int bb = 0; bb < nb; ++bb) m = fmax(m, part[bb]);
-        result[0] = m;
+        result[0] = (double)(float)m;  // the reference returns Real
     }
 }
 

@@ -1,0 +1,160 @@
+"""GPU checks of the remaining entry points (through the C-ABI): slices,
+the max-difference helpers, the CFL-substepped advection, the level-set
+projection on its own, the all-frames warp, and the context utilities."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ofblend_amd import _lib
+from ofblend_amd import manta as M
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_slice_and_place_overlap_and_out_of_range():
+    """getSliceFrom4d / placeGrid3d (grid4d.cpp:486-524): the overlap of the
+    two grids is copied, a time outside the 4D grid changes nothing."""
+    rng = np.random.default_rng(7)
+    src = rng.standard_normal((5, 6, 7, 8)).astype(np.float32)
+    s = M.Solver(gridSize=M.vec3(8, 7, 6), fourthDim=5)
+    g = s.create(M.Grid4Real)
+    g.from_numpy(src)
+    d0 = rng.standard_normal((5, 9, 6)).astype(np.float32)
+    d = M.DeviceArray.from_numpy(d0)
+    M.getSliceFrom4d(g, 3, d)
+    ref = d0.copy()
+    ref[:5, :7, :6] = src[3, :5, :7, :6]
+    assert np.array_equal(d.to_numpy(), ref)
+    M.getSliceFrom4d(g, 7, d)
+    assert np.array_equal(d.to_numpy(), ref)
+    p0 = rng.standard_normal((7, 5, 9)).astype(np.float32)
+    M.placeGrid3d(M.DeviceArray.from_numpy(p0), g, 2)
+    r4 = src.copy()
+    r4[2, :6, :5, :8] = p0[:6, :5, :8]
+    assert np.array_equal(g.to_numpy(), r4)
+    M.placeGrid3d(M.DeviceArray.from_numpy(p0), g, 9)
+    assert np.array_equal(g.to_numpy(), r4)
+
+
+def test_max_diff_real_and_vec4():
+    """grid4dMaxDiff: max |a - b| of the FP32 differences; the Vec4 form sums
+    the component differences in double (grid4d.cpp:419-464); both return
+    the reference's Real."""
+    rng = np.random.default_rng(8)
+    s = M.Solver(gridSize=M.vec3(9, 8, 7), fourthDim=6)
+    a, b = (rng.standard_normal((6, 7, 8, 9)).astype(np.float32) for _ in range(2))
+    ga, gb = s.create(M.Grid4Real), s.create(M.Grid4Real)
+    ga.from_numpy(a)
+    gb.from_numpy(b)
+    assert M.grid4dMaxDiff(ga, gb) == float(np.abs(a - b).max())
+    va, vb = (rng.standard_normal((6, 7, 8, 9, 4)).astype(np.float32) for _ in range(2))
+    gva, gvb = s.create(M.Grid4Vec4), s.create(M.Grid4Vec4)
+    gva.from_numpy(va)
+    gvb.from_numpy(vb)
+    d = np.abs(va.astype(np.float64) - vb.astype(np.float64))
+    ref = (((d[..., 0] + d[..., 1]) + d[..., 2]) + d[..., 3]).max()
+    assert M.grid4dMaxDiffVec4(gva, gvb) == float(np.float32(ref))
+
+
+@pytest.mark.parametrize("cfl", [999.0, 0.7])
+def test_advect_cfl_bit_exact(cfl):
+    """advectCflHelper4d (optflow4d.cpp:2170-2180): with cfl 0.7 the flow is
+    split into substeps; the GPU equals the oracle bit for bit."""
+    X, Y, Z, T = 11, 10, 9, 8
+    rng = np.random.default_rng(9)
+    vel = (rng.standard_normal((T, Z, Y, X, 4)) * 1.3).astype(np.float32)
+    grid = rng.standard_normal((T, Z, Y, X)).astype(np.float32)
+    s = M.Solver(gridSize=M.vec3(X, Y, Z), fourthDim=T)
+    gv, gg = s.create(M.Grid4Vec4), s.create(M.Grid4Real)
+    gv.from_numpy(vel)
+    gg.from_numpy(grid)
+    L = _lib.lib()
+    say("advect cfl", cfl)
+    _lib.check(L.flof_advect_cfl4d(gv.h, gg.h, cfl, 1.0, 1.0), gv.ctx)
+    ref = O.advect_cfl4d(vel, grid, cfl=cfl, velFactor=1.0, dt=1.0)
+    assert np.array_equal(gg.to_numpy(), ref)
+
+
+def test_corr_vels_of4d_planar_and_oracle():
+    """corrVelsOf4d on its own: planar level sets offset by a known distance
+    give the offset along the normal, and the field equals the oracle's."""
+    X, Y, Z, T = 16, 10, 10, 10
+    a, b = 7.3, 5.1
+    x = np.arange(X, dtype=np.float64) + 0.5
+    phiOrg = np.broadcast_to(x - a, (T, Z, Y, X)).astype(np.float32).copy()
+    phiTgt = np.broadcast_to(x - b, (T, Z, Y, X)).astype(np.float32).copy()
+    vel0 = np.zeros((T, Z, Y, X, 4), np.float32)
+    s = M.Solver(gridSize=M.vec3(X, Y, Z), fourthDim=T)
+    dst, vel = s.create(M.Grid4Vec4), s.create(M.Grid4Vec4)
+    vel.from_numpy(vel0)
+    po, pc, pt = s.create(M.Grid4Real), s.create(M.Grid4Real), s.create(M.Grid4Real)
+    po.from_numpy(phiOrg)
+    pc.from_numpy(phiOrg)
+    pt.from_numpy(phiTgt)
+    M.corrVelsOf4d(dst, vel, po, pc, pt, threshPhi=4.0, postVelBlur=0.0, maxIter=40)
+    rd, rv = O.corr_vels_of4d(vel0, phiOrg, phiTgt, threshPhi=4.0, postVelBlur=0.0,
+                              maxIter=40)
+    gd = dst.to_numpy()
+    core = gd[2:-2, 2:-2, 2:-2, 4:-4]
+    assert np.abs(core[..., 1:]).max() == 0.0
+    assert np.abs(core[..., 0] - (a - b)).max() <= 0.125
+    d = np.abs(gd - rd).max()
+    say("corrVelsOf4d gpu vs oracle max diff", d)
+    assert d == 0.0
+    assert np.array_equal(vel.to_numpy(), rv)
+
+
+def test_advect_time_slices_opt_all_frames_bit_exact():
+    """flof_advect_time_slices_opt: nframes output frames in one launch, each
+    equal to the oracle's loadAdvectTimeSlice_OptRun at its time."""
+    N, T, F, nf = 10, 8, 2, 5
+    rng = np.random.default_rng(10)
+    frames = O.drop_sdf(N, N, N, T, seed=3, scale=F)
+    phi_np = O.place_frames((T, N * F, N * F, N * F), frames)
+    s = M.Solver(gridSize=M.vec3(N * F, N * F, N * F), fourthDim=T)
+    phi = s.create(M.Grid4Real)
+    phi.from_numpy(phi_np)
+    sd = M.Solver(gridSize=M.vec3(N, N, N), fourthDim=T)
+    dv = (rng.standard_normal((T, N, N, N, 4)) * 1.2).astype(np.float32)
+    defo = sd.create(M.Grid4Vec4)
+    defo.from_numpy(dv)
+    dst0 = rng.standard_normal((nf, N * F, N * F, N * F)).astype(np.float32)
+    dst = M.DeviceArray.from_numpy(dst0)
+    f4 = lambda *v: (C.c_float * 4)(*v)
+    L = _lib.lib()
+    t0, dt = 0.5, 1.25
+    _lib.check(L.flof_advect_time_slices_opt(
+        defo.h, dst.ptr, N * F, N * F, N * F, nf, t0, dt, phi.h, 0.9, 1.0,
+        f4(0, 0, 0, 0), f4(1, 1, 1, 1), f4(2, 2, 2, 1), 1, 1.0), defo.ctx)
+    out = dst.to_numpy()
+    for f in range(nf):
+        ref = O.warp_slice(dv, phi_np, dst0[f], t0 + f * dt, blendAlpha=0.9,
+                           defoFactor=(2, 2, 2, 1))
+        assert np.array_equal(out[f], ref), f
+
+
+def test_context_utilities():
+    """A second context on the same device: its stream, grids with their
+    dims and device address, trimming the pool, destruction."""
+    L = _lib.lib()
+    ctx = C.c_void_p()
+    assert L.flof_ctx_create(C.byref(ctx), 0) == 0
+    try:
+        assert L.flof_ctx_stream(ctx)
+        g = C.c_void_p()
+        assert L.flof_grid4d_create(ctx, 5, 4, 3, 2, 1, 0, C.byref(g)) == 0
+        dims = (C.c_int * 4)()
+        elem, dtype = C.c_int(), C.c_int()
+        assert L.flof_grid4d_dims(g, dims, C.byref(elem), C.byref(dtype)) == 0
+        assert list(dims) == [5, 4, 3, 2] and elem.value == 1 and dtype.value == 0
+        assert L.flof_grid4d_device_ptr(g)
+        assert L.flof_grid4d_destroy(g) == 0
+        assert L.flof_ctx_trim(ctx) == 0
+    finally:
+        assert L.flof_ctx_destroy(ctx) == 0
