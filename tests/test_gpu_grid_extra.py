@@ -158,3 +158,36 @@ def test_context_utilities():
         assert L.flof_ctx_trim(ctx) == 0
     finally:
         assert L.flof_ctx_destroy(ctx) == 0
+
+
+def test_reductions_mult_and_vec4_region():
+    """getMax / getMin / getMaxAbs (grid4d.cpp:266-290: Real values; Vec4 the
+    square root of the max / min squared norm summed x, y, z, t in FP32),
+    mult (grid4d.cpp:256-259) and setRegion4dVec4 (grid4d.cpp:466-483)."""
+    rng = np.random.default_rng(12)
+    shp = (5, 6, 7, 8)
+    s = M.Solver(gridSize=M.vec3(8, 7, 6), fourthDim=5)
+    a = rng.standard_normal(shp).astype(np.float32)
+    g = s.create(M.Grid4Real)
+    g.from_numpy(a)
+    assert g.getMax() == float(a.max()) and g.getMin() == float(a.min())
+    assert g.getMaxAbs() == float(max(abs(a.min()), abs(a.max())))
+    v = rng.standard_normal(shp + (4,)).astype(np.float32)
+    gv = s.create(M.Grid4Vec4)
+    gv.from_numpy(v)
+    f = np.float32
+    ns = ((v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1]) + v[..., 2] * v[..., 2]) \
+        + v[..., 3] * v[..., 3]
+    assert ns.dtype == np.float32
+    assert gv.getMax() == float(np.sqrt(f(ns.max())))
+    assert gv.getMin() == float(np.sqrt(f(ns.min())))
+    assert gv.getMaxAbs() == gv.getMax()
+    b = rng.standard_normal(shp).astype(np.float32)
+    gb = s.create(M.Grid4Real)
+    gb.from_numpy(b)
+    g.mult(gb)
+    assert np.array_equal(g.to_numpy(), a * b)
+    M.setRegion4dVec4(gv, M.Vec4(1, 2, 1, 0), M.Vec4(5, 4, 3, 2), M.Vec4(1, 2, 3, 4))
+    ref = v.copy()
+    ref[0:3, 1:4, 2:5, 1:6] = np.float32([1, 2, 3, 4])
+    assert np.array_equal(gv.to_numpy(), ref)
