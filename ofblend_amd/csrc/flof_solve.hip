@@ -1241,57 +1241,48 @@ __global__ void k_of_finish(const float4 *__restrict__ x, float4 *__restrict__ o
 }
 
 // ---------------------------------------------------------------------------
-// Gaussian blur (optflow4d.cpp:127-174).  The reference sums a normalised
-// (2s+1)^4 window clipped to the grid; its weights factor per dimension, so
-// one pass is evaluated here as four normalised 1D passes (x, y, z over the
-// whole grid, t writing the interior only).  Rounding differs from the 4D
-// sum by a few ulp.
+// Gaussian blur (optflow4d.cpp:127-174): every interior cell sums the
+// (2s+1)^4 window clipped to the grid in the reference's order (t, z, y, x
+// from low to high), the weights and the values in FP32, then divides by
+// the weight sum.  The weight of a squared distance d is
+// (float)exp(-d / (2 sigma^2)) evaluated in double on the host, one table
+// entry per d in [0, 4 s^2], so the device uses the host's exp.
 template <class T>
-__global__ void k_blur1d(const T *__restrict__ src, T *__restrict__ dst, Dims4 D,
-                         Coord4 C, int axis, int s, const float *__restrict__ w,
-                         int interior_only)
+__global__ void __launch_bounds__(256)
+k_blur4d(const T *__restrict__ src, T *__restrict__ dst, Dims4 D, Coord4 C, int s,
+         const float *__restrict__ wt, float eps)
 {
     const int64_t n = (int64_t)D.st * D.T;
-    const int64_t stride[4] = {1, (int64_t)D.X, D.sz, D.st};
-    const int size[4] = {D.X, D.Y, D.Z, D.T};
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        int ijkt[4];
-        decompose((uint32_t)c, C, ijkt[0], ijkt[1], ijkt[2], ijkt[3]);
-        if (interior_only && !interior(D, ijkt[0], ijkt[1], ijkt[2], ijkt[3])) continue;
-        const int pos = ijkt[axis];
+    FOR_XCD(c, n)
+    {
+        int i, j, k, t;
+        decompose((uint32_t)c, C, i, j, k, t);
+        if (!interior(D, i, j, k, t)) continue;
         T val = VOps<T>::zero();
-        float wsum = 0.f;
-        for (int o = -s; o <= s; ++o) {
-            const int q = pos + o;
-            if (q < 0 || q >= size[axis]) continue;
-            const float wo = w[o + s];
-            wsum = wsum + wo;
-            val = VOps<T>::add(val, VOps<T>::muls(src[c + (int64_t)o * stride[axis]], wo));
+        float weight = 0.f;
+        for (int vt = max(t - s, 0); vt <= min(t + s, D.T - 1); ++vt)
+            for (int zk = max(k - s, 0); zk <= min(k + s, D.Z - 1); ++zk)
+                for (int yj = max(j - s, 0); yj <= min(j + s, D.Y - 1); ++yj) {
+                    const int dtzy = (vt - t) * (vt - t) + (zk - k) * (zk - k) +
+                                     (yj - j) * (yj - j);
+                    const T *row = src + (int64_t)D.st * vt + D.sz * zk + (int64_t)D.X * yj;
+                    for (int xi = max(i - s, 0); xi <= min(i + s, D.X - 1); ++xi) {
+                        const float w = wt[dtzy + (xi - i) * (xi - i)];
+                        weight = weight + w;
+                        val = VOps<T>::add(val, VOps<T>::muls(row[xi], w));
+                    }
+                }
+        if (weight > eps) {
+            if constexpr (sizeof(T) == 16) {
+                const float4 v = *(const float4 *)&val;
+                ((float4 *)dst)[c] = make_float4(v.x / weight, v.y / weight, v.z / weight,
+                                                 v.w / weight);
+            } else {
+                ((float *)dst)[c] = *(const float *)&val / weight;
+            }
+        } else {
+            dst[c] = src[c];
         }
-        dst[c] = VOps<T>::muls(val, 1.f / wsum);
-    }
-}
-
-// one pass: dst interior = G(src); dst boundary untouched
-static void blur_pass(flof_ctx *ctx, const void *src, void *dst, void *t1, void *t2,
-                      const Dims4 &D, bool vec4, int s, const float *wdev)
-{
-    const int64_t n = (int64_t)D.st * D.T;
-    Coord4 C = make_coord4(D);
-    const int nb = grid_blocks(ctx, n);
-    const void *in[4] = {src, t1, t2, t1};
-    void *out[4] = {t1, t2, t1, dst};
-    for (int axis = 0; axis < 4; ++axis) {
-        if (vec4)
-            hipLaunchKernelGGL(k_blur1d<float4>, dim3(nb), dim3(256), 0, ctx->stream,
-                               (const float4 *)in[axis], (float4 *)out[axis], D, C, axis,
-                               s, wdev, axis == 3 ? 1 : 0);
-        else
-            hipLaunchKernelGGL(k_blur1d<float>, dim3(nb), dim3(256), 0, ctx->stream,
-                               (const float *)in[axis], (float *)out[axis], D, C, axis, s,
-                               wdev, axis == 3 ? 1 : 0);
-        check_launch("k_blur1d");
     }
 }
 
@@ -1300,22 +1291,39 @@ void gaussian_blur(flof_ctx *ctx, void *a, const Dims4 &D, bool vec4, float sigm
 {
     int s = int(1. * sigma + 0.5);
     if (s == 0) s = 1;
-    FLOF_REQUIRE(s <= 64, FLOF_ERR_RANGE, "gaussianBlur: sigma too large");
-    float wh[129];
-    for (int o = -s; o <= s; ++o)
-        wh[o + s] = (float)std::exp(-(double)(float)(o * o) /
-                                    (2. * (double)sigma * (double)sigma));
+    FLOF_REQUIRE(s <= 16, FLOF_ERR_RANGE, "gaussianBlur: sigma too large");
+    // weights by squared distance d (the reference forms d as a float sum of
+    // squared integer offsets: exact)
+    const int nw = 4 * s * s + 1;
+    std::vector<float> wh(nw);
+    for (int d = 0; d < nw; ++d)
+        wh[d] = (float)std::exp(-(double)(float)d / (2. * (double)sigma * (double)sigma));
     const int64_t n = (int64_t)D.st * D.T;
     const size_t bytes = (size_t)n * (vec4 ? 16 : 4);
-    Tmp wdev(ctx, 129 * sizeof(float)), b(ctx, bytes), t1(ctx, bytes), t2(ctx, bytes);
-    FLOF_HIP(hipMemcpyAsync(wdev.p, wh, sizeof(wh), hipMemcpyHostToDevice, ctx->stream));
+    Tmp wdev(ctx, (size_t)nw * sizeof(float)), b(ctx, bytes);
+    FLOF_HIP(hipMemcpyAsync(wdev.p, wh.data(), (size_t)nw * sizeof(float),
+                            hipMemcpyHostToDevice, ctx->stream));
     launch_fill(ctx, b.p, bytes);
+    Coord4 C = make_coord4(D);
+    const int nb = grid_blocks_xcd(ctx, n);
+    const float eps = 1e-6f;  // VECTOR_EPSILON of FP32 builds
+    auto pass = [&](const void *src, void *dst) {
+        if (vec4)
+            hipLaunchKernelGGL(k_blur4d<float4>, dim3(nb), dim3(256), 0, ctx->stream,
+                               (const float4 *)src, (float4 *)dst, D, C, s,
+                               (const float *)wdev.as<float>(), eps);
+        else
+            hipLaunchKernelGGL(k_blur4d<float>, dim3(nb), dim3(256), 0, ctx->stream,
+                               (const float *)src, (float *)dst, D, C, s,
+                               (const float *)wdev.as<float>(), eps);
+        check_launch("k_blur4d");
+    };
     for (int it = 0; it < iter; ++it) {
-        // pass 1 into a zeroed temp, pass 2 back into the grid: the grid keeps
-        // its outer layer, the second pass sees a zero outer layer
-        blur_pass(ctx, a, b.p, t1.p, t2.p, D, vec4, s, wdev.as<float>());
-        blur_pass(ctx, b.p, a, t1.p, t2.p, D, vec4, s, wdev.as<float>());
-        if (it + 1 < iter) launch_fill(ctx, b.p, bytes);
+        // pass 1 into the temporary (its outer layer stays zero), pass 2 back
+        // into the grid, which keeps its own outer layer: the reference's two
+        // swaps with a zeroed temporary
+        pass(a, b.p);
+        pass(b.p, a);
     }
     // the host weights must outlive the async copy
     FLOF_HIP(hipStreamSynchronize(ctx->stream));
