@@ -201,8 +201,8 @@ int flof_frame_set_bound(flof_ctx *ctx, float *a, int X, int Y, int Z, float val
  * pool: FLOF_MIX_UPDATE reads 3 float4 + 1 float and writes 2 float4 per
  * cell (84 B), FLOF_MIX_DIR reads 2 float4 + 1 float and writes 1 float4
  * (52 B), FLOF_MIX_COPY reads and writes 1 float4 (32 B); non-temporal
- * accesses as in the PCG kernels, best of two launch shapes, each timed
- * over `reps` launches.  *gbps = bytes moved / time, in GB/s. */
+ * accesses, best of 1, 2, 4, 8 and 16 workgroups of 256 threads per CU,
+ * each timed over `reps` launches.  *gbps = bytes moved / time, in GB/s. */
 enum flof_mix { FLOF_MIX_UPDATE = 0, FLOF_MIX_DIR = 1, FLOF_MIX_COPY = 2 };
 int flof_stream_bandwidth(flof_ctx *ctx, int mix, int64_t cells, int reps, double *gbps);
 

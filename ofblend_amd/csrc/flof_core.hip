@@ -1013,8 +1013,8 @@ k_stream_update(float4 *__restrict__ x, float4 *__restrict__ r, const float4 *__
 {
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
          c += (int64_t)gridDim.x * blockDim.x) {
-        const float4 pc = p[c];
-        const float gc = g[c];
+        const float4 pc = nt_ld(p + c);
+        const float gc = __builtin_nontemporal_load(g + c);
         float4 xc = nt_ld(x + c), rc = nt_ld(r + c);
         xc = make_float4(xc.x + a * pc.x, xc.y + a * pc.y, xc.z + a * pc.z, xc.w + a * pc.w);
         rc = make_float4(rc.x - gc * pc.x, rc.y - gc * pc.y, rc.z - gc * pc.z, rc.w - gc * pc.w);
@@ -1030,7 +1030,7 @@ k_stream_dir(float4 *__restrict__ p, const float4 *__restrict__ r, const float *
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
          c += (int64_t)gridDim.x * blockDim.x) {
         const float4 rc = nt_ld(r + c), pc = nt_ld(p + c);
-        const float gc = g[c];
+        const float gc = __builtin_nontemporal_load(g + c);
         nt_st(p + c, make_float4(rc.x * gc + b * pc.x, rc.y * gc + b * pc.y,
                                  rc.z * gc + b * pc.z, rc.w * gc + b * pc.w));
     }
@@ -1063,7 +1063,9 @@ int flof_stream_bandwidth(flof_ctx *ctx, int mix, int64_t cells, int reps, doubl
     FLOF_HIP(hipEventCreate(&e1));
     double best = 0.0;
     hipError_t err = hipSuccess;
-    for (int bpc : {4, 8}) {
+    // the best over the workgroups per CU: which count HBM serves best
+    // depends on the mix (2 for the update's, measured on MI355X)
+    for (int bpc : {1, 2, 4, 8, 16}) {
         const int grid = ctx->num_cu * bpc;
         auto launch = [&]() {
             if (mix == 0)

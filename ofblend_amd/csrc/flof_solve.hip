@@ -1073,6 +1073,17 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
         }
         load_halo(t0);
         const int wl = wy * LT_X + lane;
+        // the update's streamed x and r: this step's in xo / ro, the next
+        // step's requested while this one computes
+        float4 xo[LT_Z], ro[LT_Z];
+        if (UPDATE) {
+#pragma unroll
+            for (int q = 0; q < LT_Z; ++q) {
+                const int c = c3 + sz * (z0 + q) + stt * t0;
+                xo[q] = own[q] ? ld_stream(x + c) : zero4;
+                ro[q] = own[q] ? ld_stream(r + c) : zero4;
+            }
+        }
         for (int t = t0; t < t1; ++t) {
             const int ct = stt * t;
 #pragma unroll
@@ -1093,15 +1104,14 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
             szi[0][wl] = zm_i;
             szp[1][wl] = f4scale(od, zp_v);
             szi[1][wl] = zp_i;
-            // the streamed x and r of this step first (waited for in the
-            // compute), then the prefetch of the next step
-            float4 xo[LT_Z], ro[LT_Z];
-            if (UPDATE) {
+            // prefetch of the next step: x and r, the halo, own p at t+2
+            float4 xq[LT_Z], rq[LT_Z];
+            if (UPDATE && t + 1 < t1) {
 #pragma unroll
                 for (int q = 0; q < LT_Z; ++q) {
-                    const int c = c3 + sz * (z0 + q) + ct;
-                    xo[q] = own[q] ? ld_stream(x + c) : zero4;
-                    ro[q] = own[q] ? ld_stream(r + c) : zero4;
+                    const int c = c3 + sz * (z0 + q) + ct + stt;
+                    xq[q] = own[q] ? ld_stream(x + c) : zero4;
+                    rq[q] = own[q] ? ld_stream(r + c) : zero4;
                 }
             }
             if (t + 1 < t1) {
@@ -1196,6 +1206,13 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
 #pragma unroll
             for (int q = 0; q < LT_Z; ++q) pms[q] = sp[lslot(q, wy, lane)];
             __syncthreads();
+            if (UPDATE && t + 1 < t1) {
+#pragma unroll
+                for (int q = 0; q < LT_Z; ++q) {
+                    xo[q] = xq[q];
+                    ro[q] = rq[q];
+                }
+            }
 #pragma unroll
             for (int q = 0; q < LT_Z; ++q) {
                 pc[q] = pp[q];
