@@ -1016,7 +1016,14 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
         for (int q = 0; q < LT_Z; ++q) own[q] = in_xy && z0 + q < D.Z;
         const bool has_zm = in_xy && z0 >= 1;
         const bool has_zp = in_xy && z0 + LT_Z < D.Z;
+        // Loads are unconditional: where the cell of a lane lies outside the
+        // grid (x, y, z, or t beyond the last step) the lane reads cell 0
+        // instead.  Such values only reach registers and LDS slots that no
+        // computation reads: only interior cells read neighbours, and the 8
+        // neighbours of an interior cell lie inside the grid.  The loads then
+        // need no exec-mask branches (p.Ap at 64^4: 92 -> 80 us).
         const int czm = c3 + sz * (z0 - 1), czp = c3 + sz * (z0 + LT_Z);
+        const int czm_l = has_zm ? czm : 0, czp_l = has_zp ? czp : 0;
         // y-halo rows y0-1 / y0+4 of the 4 z planes: 512 cells, 2 per thread
         const int hy_side = tid & 1, hy_q = (tid >> 1) & 3, hy_lane = tid >> 3;
         const int hy_row = hy_side ? LT_Y : -1;
@@ -1047,15 +1054,15 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
             const int ct = stt * t;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                hy_v[h] = hy_ok[h] ? PS::ld(p, hy_c[h] + ct) : zero4;
-                hy_i[h] = hy_ok[h] ? i1[hy_c[h] + ct] : 0.f;
+                hy_v[h] = PS::ld(p, hy_c[h] + ct);
+                hy_i[h] = i1[hy_c[h] + ct];
             }
             hx_v = hx_ok ? PS::ld(p, hx_c + ct) : zero4;
             hx_i = hx_ok ? i1[hx_c + ct] : 0.f;
-            zm_v = has_zm ? PS::ld(p, czm + ct) : zero4;
-            zm_i = has_zm ? i1[czm + ct] : 0.f;
-            zp_v = has_zp ? PS::ld(p, czp + ct) : zero4;
-            zp_i = has_zp ? i1[czp + ct] : 0.f;
+            zm_v = PS::ld(p, czm_l + ct);
+            zm_i = i1[czm_l + ct];
+            zp_v = PS::ld(p, czp_l + ct);
+            zp_i = i1[czp_l + ct];
         };
         // own cells: offd * p at t-1; p at t, t+1 and (prefetched) t+2
         float4 pms[LT_Z], pc[LT_Z], pp[LT_Z], pn[LT_Z];
@@ -1064,12 +1071,12 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
         for (int q = 0; q < LT_Z; ++q) {
             const int c = c3 + sz * (z0 + q) + stt * t0;
             const bool hm = own[q] && t0 > 0, hp = own[q] && t0 + 1 < D.T;
-            pms[q] = hm ? f4scale(od, PS::ld(p, c - stt)) : zero4;
-            im[q] = hm ? i1[c - stt] : 0.f;
-            pc[q] = own[q] ? PS::ld(p, c) : zero4;
-            ic[q] = own[q] ? i1[c] : 0.f;
-            pp[q] = hp ? PS::ld(p, c + stt) : zero4;
-            ip[q] = hp ? i1[c + stt] : 0.f;
+            pms[q] = f4scale(od, PS::ld(p, hm ? c - stt : 0));
+            im[q] = i1[hm ? c - stt : 0];
+            pc[q] = PS::ld(p, own[q] ? c : 0);
+            ic[q] = i1[own[q] ? c : 0];
+            pp[q] = PS::ld(p, hp ? c + stt : 0);
+            ip[q] = i1[hp ? c + stt : 0];
         }
         load_halo(t0);
         const int wl = wy * LT_X + lane;
@@ -1079,9 +1086,9 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
         if (UPDATE) {
 #pragma unroll
             for (int q = 0; q < LT_Z; ++q) {
-                const int c = c3 + sz * (z0 + q) + stt * t0;
-                xo[q] = own[q] ? ld_stream(x + c) : zero4;
-                ro[q] = own[q] ? ld_stream(r + c) : zero4;
+                const int c = own[q] ? c3 + sz * (z0 + q) + stt * t0 : 0;
+                xo[q] = ld_stream(x + c);
+                ro[q] = ld_stream(r + c);
             }
         }
         for (int t = t0; t < t1; ++t) {
@@ -1109,9 +1116,9 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
             if (UPDATE && t + 1 < t1) {
 #pragma unroll
                 for (int q = 0; q < LT_Z; ++q) {
-                    const int c = c3 + sz * (z0 + q) + ct + stt;
-                    xq[q] = own[q] ? ld_stream(x + c) : zero4;
-                    rq[q] = own[q] ? ld_stream(r + c) : zero4;
+                    const int c = own[q] ? c3 + sz * (z0 + q) + ct + stt : 0;
+                    xq[q] = ld_stream(x + c);
+                    rq[q] = ld_stream(r + c);
                 }
             }
             if (t + 1 < t1) {
@@ -1119,9 +1126,9 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
 #pragma unroll
                 for (int q = 0; q < LT_Z; ++q) {
                     const bool hn = own[q] && t + 2 < D.T;
-                    const int c = c3 + sz * (z0 + q) + ct + 2 * stt;
-                    pn[q] = hn ? PS::ld(p, c) : zero4;
-                    in[q] = hn ? i1[c] : 0.f;
+                    const int c = hn ? c3 + sz * (z0 + q) + ct + 2 * stt : 0;
+                    pn[q] = PS::ld(p, c);
+                    in[q] = i1[c];
                 }
             }
             __syncthreads();
