@@ -2290,8 +2290,10 @@ int flof_optical_flow_multiscale4d_batch(flof_ctx *ctx, int n, flof_grid4d *cons
         for (flof_grid4d *g : {vel[j], i0[j], i1[j]})
             if (g->ctx != ctx) FLOF_HIP(hipStreamSynchronize(g->ctx->stream));
     std::atomic<int> next{0};
-    std::vector<int> rc(n, FLOF_OK);
-    std::vector<std::string> msg(n);
+    // a solve no worker took (every worker failed to select the device)
+    // reports that instead of success
+    std::vector<int> rc(n, FLOF_ERR_HIP);
+    std::vector<std::string> msg(n, "not run: no worker could select the device");
     auto run = [&](flof_ctx *w) {
         if (hipSetDevice(w->device) != hipSuccess) return;
         for (;;) {
@@ -2308,6 +2310,7 @@ int flof_optical_flow_multiscale4d_batch(flof_ctx *ctx, int n, flof_grid4d *cons
                                            dims_of(vel[j]), P, p->level, 1.f, inf, nullptr);
                 FLOF_HIP(hipStreamSynchronize(w->stream));
                 if (errs) errs[j] = e;
+                rc[j] = FLOF_OK;
             } catch (const flof::Error &e) {
                 rc[j] = e.code;
                 msg[j] = e.msg;
