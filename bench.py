@@ -34,9 +34,14 @@ PARAMS = dict(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5, minGridSize=16,
 FIXED_RESIDUAL_CAP = 20000
 # algorithmic bytes per cell of one launch (FP32 Vec4 = 16 B per vector,
 # i1 = 4 B; g and diag^-1 are recomputed from i1)
-B_UPDATE = 16 * 3 + 4 + 16 * 2   # read p,r,x,i1; write r,x
+B_UPDATE = 16 * 2 + 4 + 16       # read p,r,i1; write r
 B_APPLY = 16 + 4                 # read p,i1 (p.Ap only)
-B_DIR = 16 * 2 + 4 + 16          # read r,p,i1; write p
+# the direction kernel moves x every other iteration (the x update of an even
+# iteration is deferred to the next one): even read r,p,i1, write p; odd also
+# read x and the previous direction, write x
+B_DIR_EVEN = 16 * 2 + 4 + 16
+B_DIR_ODD = 16 * 4 + 4 + 16 * 2
+B_DIR = (B_DIR_EVEN + B_DIR_ODD) / 2  # per iteration
 B_ITER_MIN = 112                 # x,r,p read+written once, g read once
 # kernel names of the PCG launch kinds on the finest level: the library runs
 # the LDS-tiled stencil kernels there (64^4, FP32: long t segments of full
@@ -234,20 +239,29 @@ def main():
     # what HBM sustains for the byte mixes of the PCG kernels when a plain
     # streaming kernel moves them (no stencil work), on the finest level's
     # cell count: the practical ceiling the kernels are compared with
-    from ofblend_amd.abi import FLOF_MIX_COPY, FLOF_MIX_DIR, FLOF_MIX_UPDATE
+    from ofblend_amd.abi import FLOF_MIX_COPY, FLOF_MIX_DIR, FLOF_MIX_DIR_X
     stream = {}
-    for nm, mix in (("update_mix_84B", FLOF_MIX_UPDATE), ("dir_mix_52B", FLOF_MIX_DIR),
+    for nm, mix in (("mix_52B", FLOF_MIX_DIR), ("mix_100B", FLOF_MIX_DIR_X),
                     ("copy_32B", FLOF_MIX_COPY)):
         g = C.c_double()
         rc = L.flof_stream_bandwidth(ctx, mix, cells0, 20, C.byref(g))
         if rc == 0:
             stream[nm] = g.value
     out["stream_bandwidth_GBps"] = stream
-    same = {KN_UPDATE: "update_mix_84B", KN_DIR: "dir_mix_52B"}.get(dom)
-    if same in stream:
+    out["stream_bandwidth_note"] = ("52 B: read 2 float4 + 1 float, write 1 float4 (the "
+                                    "update and the even direction launches); 100 B: read "
+                                    "4 float4 + 1 float, write 2 float4 (the odd ones)")
+    ach = None
+    if dom == KN_UPDATE and "mix_52B" in stream:
+        ach = stream["mix_52B"]
+    elif dom == KN_DIR and "mix_52B" in stream and "mix_100B" in stream:
+        # an even and an odd launch at the stream rates of their mixes
+        ach = (B_DIR_EVEN + B_DIR_ODD) / (B_DIR_EVEN / stream["mix_52B"] +
+                                          B_DIR_ODD / stream["mix_100B"])
+    if ach:
         roofline["achievable_same_mix"] = {
-            "GBps": stream[same], "frac": achieved / 1e9 / stream[same],
-            "note": "a streaming kernel with the same bytes per cell and no stencil work"}
+            "GBps": ach, "frac": achieved / 1e9 / ach,
+            "note": "streaming kernels with the same bytes per cell and no stencil work"}
 
     # host-buffer boundary: the inputs come from and the flow goes to host
     # memory (pageable numpy arrays): copies around one solve

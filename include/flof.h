@@ -127,8 +127,8 @@ int flof_ctx_trim(flof_ctx *ctx);
  * flof_ctx_timing returns the number of launches and their summed duration. */
 enum flof_kernel_kind {
     FLOF_KT_CG_APPLY = 0,   /* p.(A p)                                 */
-    FLOF_KT_CG_UPDATE = 1,  /* x, r update, signed max of r, z.r       */
-    FLOF_KT_CG_DIR = 2,     /* p = z + beta p                          */
+    FLOF_KT_CG_UPDATE = 1,  /* r update, signed max of r, z.r         */
+    FLOF_KT_CG_DIR = 2,     /* x update, p = z + beta p                */
     FLOF_KT_WARP = 3,       /* 3D frame warp                           */
     FLOF_KT_ADVECT = 4,     /* 4D semi-Lagrangian advection            */
     FLOF_KT_COUNT = 8
@@ -199,11 +199,19 @@ int flof_frame_set_bound(flof_ctx *ctx, float *a, int X, int Y, int Z, float val
  * streaming kernel reaches with the byte mix of one PCG kernel and no
  * stencil work, on `cells` cells of scratch memory taken from the context's
  * pool: FLOF_MIX_UPDATE reads 3 float4 + 1 float and writes 2 float4 per
- * cell (84 B), FLOF_MIX_DIR reads 2 float4 + 1 float and writes 1 float4
- * (52 B), FLOF_MIX_COPY reads and writes 1 float4 (32 B); non-temporal
+ * cell (84 B: an update that also moves x), FLOF_MIX_DIR reads 2 float4 +
+ * 1 float and writes 1 float4 (52 B: the update and the direction kernel of
+ * even iterations), FLOF_MIX_COPY reads and writes 1 float4 (32 B),
+ * FLOF_MIX_DIR_X reads 4 float4 + 1 float and writes 2 float4 (100 B: the
+ * direction kernel of odd iterations, which also moves x); non-temporal
  * accesses, best of 1, 2, 4, 8 and 16 workgroups of 256 threads per CU,
  * each timed over `reps` launches.  *gbps = bytes moved / time, in GB/s. */
-enum flof_mix { FLOF_MIX_UPDATE = 0, FLOF_MIX_DIR = 1, FLOF_MIX_COPY = 2 };
+enum flof_mix {
+    FLOF_MIX_UPDATE = 0,
+    FLOF_MIX_DIR = 1,
+    FLOF_MIX_COPY = 2,
+    FLOF_MIX_DIR_X = 3
+};
 int flof_stream_bandwidth(flof_ctx *ctx, int mix, int64_t cells, int reps, double *gbps);
 
 /* ---- Grid4d storage (source/grid4d.h:117-265, source/grid4d.cpp:60-138) - */

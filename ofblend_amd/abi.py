@@ -27,6 +27,7 @@ FLOF_STENCIL_QARRAY = 3
 FLOF_MIX_UPDATE = 0
 FLOF_MIX_DIR = 1
 FLOF_MIX_COPY = 2
+FLOF_MIX_DIR_X = 3
 
 FLOF_FRAME_COPY = 0
 FLOF_FRAME_SET_CONST = 1

@@ -1035,6 +1035,23 @@ k_stream_dir(float4 *__restrict__ p, const float4 *__restrict__ r, const float *
                                  rc.z * gc + b * pc.z, rc.w * gc + b * pc.w));
     }
 }
+// direction mix of odd iterations: read x, r, p, p_prev, g; write x, p
+// (100 B per cell)
+__global__ void __launch_bounds__(256)
+k_stream_dirx(float4 *__restrict__ x, float4 *__restrict__ p, const float4 *__restrict__ r,
+              const float4 *__restrict__ pc, const float *__restrict__ g, int64_t n, float b)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const float4 rc = nt_ld(r + c), pk = nt_ld(pc + c), pp = nt_ld(p + c);
+        const float4 xc = nt_ld(x + c);
+        const float gc = __builtin_nontemporal_load(g + c);
+        nt_st(x + c, make_float4(xc.x + b * pp.x + b * pk.x, xc.y + b * pp.y + b * pk.y,
+                                 xc.z + b * pp.z + b * pk.z, xc.w + b * pp.w + b * pk.w));
+        nt_st(p + c, make_float4(rc.x * gc + b * pk.x, rc.y * gc + b * pk.y,
+                                 rc.z * gc + b * pk.z, rc.w * gc + b * pk.w));
+    }
+}
 // copy: read a, write b (32 B per cell)
 __global__ void __launch_bounds__(256)
 k_stream_copy(float4 *__restrict__ b, const float4 *__restrict__ a, int64_t n)
@@ -1049,15 +1066,21 @@ k_stream_copy(float4 *__restrict__ b, const float4 *__restrict__ a, int64_t n)
 int flof_stream_bandwidth(flof_ctx *ctx, int mix, int64_t cells, int reps, double *gbps)
 {
     API_TRY
-    FLOF_REQUIRE(ctx && mix >= 0 && mix <= 2 && cells > 0 && reps > 0 && gbps, FLOF_ERR_ARG,
+    FLOF_REQUIRE(ctx && mix >= 0 && mix <= 3 && cells > 0 && reps > 0 && gbps, FLOF_ERR_ARG,
                  "flof_stream_bandwidth: bad argument");
     const size_t v4 = (size_t)cells * 16;
-    Tmp a(ctx, v4), b(ctx, v4), c(ctx, v4), g(ctx, (size_t)cells * 4);
+    Tmp a(ctx, v4), b(ctx, v4), c(ctx, v4), d(ctx, mix == FLOF_MIX_DIR_X ? v4 : 16),
+        g(ctx, (size_t)cells * 4);
     FLOF_HIP(hipMemsetAsync(a.p, 0, v4, ctx->stream));
     FLOF_HIP(hipMemsetAsync(b.p, 0, v4, ctx->stream));
     FLOF_HIP(hipMemsetAsync(c.p, 0, v4, ctx->stream));
+    if (mix == FLOF_MIX_DIR_X) FLOF_HIP(hipMemsetAsync(d.p, 0, v4, ctx->stream));
     FLOF_HIP(hipMemsetAsync(g.p, 0, (size_t)cells * 4, ctx->stream));
-    const double bytes = (mix == 0 ? 84.0 : mix == 1 ? 52.0 : 32.0) * (double)cells;
+    const double per_cell = mix == FLOF_MIX_UPDATE ? 84.0
+                            : mix == FLOF_MIX_DIR  ? 52.0
+                            : mix == FLOF_MIX_COPY ? 32.0
+                                                   : 100.0;
+    const double bytes = per_cell * (double)cells;
     hipEvent_t e0, e1;
     FLOF_HIP(hipEventCreate(&e0));
     FLOF_HIP(hipEventCreate(&e1));
@@ -1075,6 +1098,11 @@ int flof_stream_bandwidth(flof_ctx *ctx, int mix, int64_t cells, int reps, doubl
             else if (mix == 1)
                 hipLaunchKernelGGL(k_stream_dir, dim3(grid), dim3(256), 0, ctx->stream,
                                    a.as<float4>(), (const float4 *)b.as<float4>(),
+                                   (const float *)g.as<float>(), cells, 0.5f);
+            else if (mix == FLOF_MIX_DIR_X)
+                hipLaunchKernelGGL(k_stream_dirx, dim3(grid), dim3(256), 0, ctx->stream,
+                                   a.as<float4>(), b.as<float4>(), (const float4 *)c.as<float4>(),
+                                   (const float4 *)d.as<float4>(),
                                    (const float *)g.as<float>(), cells, 0.5f);
             else
                 hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream,

@@ -15,8 +15,8 @@
 // applied from g on the fly with the identical FP32 operation order, so
 // q = A p is bit-identical to the reference's applyMat.
 //
-// PCG per iteration: three stencil kernels (p.q, the x / r update with z.r
-// and max r, the direction update; 156 B per cell) with two 1-block FP64
+// PCG per iteration: three stencil kernels (p.q, the r update with z.r and
+// max r, the x and direction update; 148 B per cell) with two 1-block FP64
 // finalizers between them (k_cg_alpha: alpha = sigma / p.q; k_cg_beta:
 // stopping test, beta).  p.q and the update run as LDS-tiled kernels
 // (k_cg_stencil_lds) on levels with long t segments and as t-marching
@@ -59,6 +59,7 @@ namespace flof {
 struct CgState {
     double sigma;
     double alpha;
+    double alpha_prev;  // alpha of the previous iteration (deferred x update)
     double beta;
     double acc;      // stopping threshold cgAccuracy * resIni
     double resIni;   // signed max of the initial residual
@@ -497,7 +498,10 @@ __global__ void k_cg_alpha(CgState *st, const DD *pd, int nb)
     DD s;
     float m;
     fin_dd_max(pd, nullptr, nb, s, m);
-    if (threadIdx.x == 0) st->alpha = st->sigma / (s.hi + s.lo);
+    if (threadIdx.x == 0) {
+        st->alpha_prev = st->alpha;
+        st->alpha = st->sigma / (s.hi + s.lo);
+    }
 }
 
 __global__ void __launch_bounds__(256)
@@ -606,9 +610,10 @@ k_cg_dir(float4 *__restrict__ x, float4 *__restrict__ p, const float4 *__restric
 // rounding is innocuous), and q = A p is formed twice, bit-identically: once
 // for p.q and once in the update.  Per cell and iteration:
 //   k_cg_pq_tm     : read p, i1                        20 B  -> alpha
-//   k_cg_update_tm : read p, i1, r, x; write r, x      84 B  -> beta
-//   k_cg_dir_tm    : read r, i1, p; write p            52 B
-// 156 B instead of 208 B of the three streaming kernels.
+//   k_cg_update_tm : read p, i1, r; write r            52 B  -> beta
+//   k_cg_dir_tm    : read r, i1, p; write p            52 B  (even iterations)
+//                    also read x, previous p; write x 100 B  (odd iterations)
+// 148 B on average instead of 208 B of the three streaming kernels.
 
 __device__ __forceinline__ float4 prec_f32(const float4 &g, float diag)
 {
@@ -838,18 +843,18 @@ k_cg_pq_tm(const typename PS::T *__restrict__ p, const float *__restrict__ i1, L
     if (threadIdx.x == 0) part_dd[blockIdx.x] = acc;
 }
 
-// x += alpha p, r -= alpha (A p) (optflow4d.cpp:309-310), z = r diag^-1,
-// z.r and the signed max of r (:311-317)
+// r -= alpha (A p) (optflow4d.cpp:310), z = r diag^-1, z.r and the signed
+// max of r (:311-317); x += alpha p (:309) is applied by k_cg_dir_tm
 template <class PS>
 __global__ void __launch_bounds__(256)
-k_cg_update_tm(float4 *__restrict__ r, float4 *__restrict__ x,
-               const typename PS::T *__restrict__ p, const float *__restrict__ i1, LevelGeom G,
-               TMarch M, float dxf, const CgState *st, DD *part_dd, float *part_max)
+k_cg_update_tm(float4 *__restrict__ r, const typename PS::T *__restrict__ p,
+               const float *__restrict__ i1, LevelGeom G, TMarch M, float dxf,
+               const CgState *st, DD *part_dd, float *part_max)
 {
     if (st->done) return;
     const Dims4 &D = G.D;
     const TmCell w = tm_cell(D, M);
-    const double alpha = st->alpha, nalpha = -st->alpha;
+    const double nalpha = -st->alpha;
     DD sig{0., 0.};
     float mx = -INFINITY;
     if (w.live) {
@@ -874,12 +879,6 @@ k_cg_update_tm(float4 *__restrict__ r, float4 *__restrict__ x,
                 qc = apply_I(pc);
                 pr = make_float4(1.f, 1.f, 1.f, 1.f);
             }
-            float4 xc = ld_stream(x + c);
-            xc.x = (float)((double)xc.x + alpha * (double)pc.x);
-            xc.y = (float)((double)xc.y + alpha * (double)pc.y);
-            xc.z = (float)((double)xc.z + alpha * (double)pc.z);
-            xc.w = (float)((double)xc.w + alpha * (double)pc.w);
-            st_stream(x + c, xc);
             float4 rc = ld_stream(r + c);
             rc.x = (float)((double)rc.x + nalpha * (double)qc.x);
             rc.y = (float)((double)rc.y + nalpha * (double)qc.y);
@@ -903,19 +902,35 @@ k_cg_update_tm(float4 *__restrict__ r, float4 *__restrict__ x,
     }
 }
 
-// p = z + beta p with z = r diag^-1 (optflow4d.cpp:318-323); x was updated
-// in k_cg_update_tm, so only dir_mode 1 has work
-template <class PS>
-__global__ void __launch_bounds__(256)
-k_cg_dir_tm(typename PS::T *__restrict__ p, const float4 *__restrict__ r,
-            const float *__restrict__ i1, LevelGeom G, TMarch M, float dxf,
-            const CgState *st)
+// x += alpha p (optflow4d.cpp:309), then p = z + beta p with z = r diag^-1
+// (:318-323) unless this was the last iteration.  The direction is double
+// buffered: p_k is read from p and p_(k+1) written to pn, which held
+// p_(k-1).  The x update of an even iteration k is deferred to the next
+// one: there (ODD) x += alpha_(k-1) p_(k-1), then x += alpha_k p_k, the same
+// two roundings in the same order, so x is the same while it is read and
+// written every other iteration only.  dir_mode 2 (the last iteration)
+// applies the pending updates and leaves p.
+__device__ __forceinline__ float4 x_plus(const float4 &x, double a, const float4 &p)
 {
-    if (st->dir_mode != 1) return;
+    return make_float4((float)((double)x.x + a * (double)p.x),
+                       (float)((double)x.y + a * (double)p.y),
+                       (float)((double)x.z + a * (double)p.z),
+                       (float)((double)x.w + a * (double)p.w));
+}
+
+template <class PS, bool ODD>
+__global__ void __launch_bounds__(256)
+k_cg_dir_tm(const typename PS::T *__restrict__ p, typename PS::T *pn, float4 *__restrict__ x,
+            const float4 *__restrict__ r, const float *__restrict__ i1, LevelGeom G, TMarch M,
+            float dxf, const CgState *st)
+{
+    const int mode = st->dir_mode;
+    if (mode != 1 && mode != 2) return;
+    const bool xupd = ODD || mode == 2;
     const Dims4 &D = G.D;
     const TmCell w = tm_cell(D, M);
     if (!w.live) return;
-    const double beta = st->beta;
+    const double alpha = st->alpha, alpha_prev = st->alpha_prev, beta = st->beta;
     const int64_t c3 = w.c3;
     float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
     float ic = i1[c3 + D.st * w.t0];
@@ -923,17 +938,24 @@ k_cg_dir_tm(typename PS::T *__restrict__ p, const float4 *__restrict__ r,
     for (int t = w.t0; t < w.t1; ++t) {
         const int64_t c = c3 + D.st * t;
         const float ip = t + 1 < D.T ? i1[c + D.st] : 0.f;
-        float4 pr = make_float4(1.f, 1.f, 1.f, 1.f);
-        if (w.sp_int && t >= 1 && t < D.T - 1)
-            pr = prec_f32(grad_i1(i1, c, D, im, ip, dxf), G.diag);
-        const float4 z = f4mul(ld_stream(r + c), pr);
         const float4 pc = PS::ld_once(p, c);
-        float4 np;
-        np.x = (float)((double)z.x + beta * (double)pc.x);
-        np.y = (float)((double)z.y + beta * (double)pc.y);
-        np.z = (float)((double)z.z + beta * (double)pc.z);
-        np.w = (float)((double)z.w + beta * (double)pc.w);
-        PS::st_once(p, c, np);
+        if (xupd) {
+            float4 xc = ld_stream(x + c);
+            if (ODD) xc = x_plus(xc, alpha_prev, PS::ld_once(pn, c));
+            st_stream(x + c, x_plus(xc, alpha, pc));
+        }
+        if (mode == 1) {
+            float4 pr = make_float4(1.f, 1.f, 1.f, 1.f);
+            if (w.sp_int && t >= 1 && t < D.T - 1)
+                pr = prec_f32(grad_i1(i1, c, D, im, ip, dxf), G.diag);
+            const float4 z = f4mul(ld_stream(r + c), pr);
+            float4 np;
+            np.x = (float)((double)z.x + beta * (double)pc.x);
+            np.y = (float)((double)z.y + beta * (double)pc.y);
+            np.z = (float)((double)z.z + beta * (double)pc.z);
+            np.w = (float)((double)z.w + beta * (double)pc.w);
+            PS::st_once(pn, c, np);
+        }
         im = ic;
         ic = ip;
     }
@@ -977,7 +999,7 @@ __device__ __forceinline__ float4 f4scale(float s, const float4 &v)
 template <class PS, bool UPDATE>
 __global__ void __launch_bounds__(256)
 k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__ i1,
-                 float4 *__restrict__ r, float4 *__restrict__ x, LevelGeom G, TileMarch M,
+                 float4 *__restrict__ r, LevelGeom G, TileMarch M,
                  float dxf, const CgState *st, DD *part_dd, float *part_max)
 {
     if (st->done) return;  // uniform over the launch
@@ -1045,7 +1067,7 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
             hx_ok = tid < 32 && hi >= 0 && hi < D.X && hj < D.Y && hk < D.Z;
             hx_c = hx_ok ? hi + D.X * hj + sz * hk : 0;
         }
-        const double alpha = st->alpha, nalpha = -st->alpha;
+        const double nalpha = -st->alpha;
         const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
         // halo of one t step in registers until it is written to LDS
         float4 hy_v[2], hx_v, zm_v, zp_v;
@@ -1080,16 +1102,13 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
         }
         load_halo(t0);
         const int wl = wy * LT_X + lane;
-        // the update's streamed x and r: this step's in xo / ro, the next
-        // step's requested while this one computes
-        float4 xo[LT_Z], ro[LT_Z];
+        // the update's streamed r: this step's in ro, the next step's
+        // requested while this one computes
+        float4 ro[LT_Z];
         if (UPDATE) {
 #pragma unroll
-            for (int q = 0; q < LT_Z; ++q) {
-                const int c = own[q] ? c3 + sz * (z0 + q) + stt * t0 : 0;
-                xo[q] = ld_stream(x + c);
-                ro[q] = ld_stream(r + c);
-            }
+            for (int q = 0; q < LT_Z; ++q)
+                ro[q] = ld_stream(r + (own[q] ? c3 + sz * (z0 + q) + stt * t0 : 0));
         }
         for (int t = t0; t < t1; ++t) {
             const int ct = stt * t;
@@ -1111,15 +1130,12 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
             szi[0][wl] = zm_i;
             szp[1][wl] = f4scale(od, zp_v);
             szi[1][wl] = zp_i;
-            // prefetch of the next step: x and r, the halo, own p at t+2
-            float4 xq[LT_Z], rq[LT_Z];
+            // prefetch of the next step: r, the halo, own p at t+2
+            float4 rq[LT_Z];
             if (UPDATE && t + 1 < t1) {
 #pragma unroll
-                for (int q = 0; q < LT_Z; ++q) {
-                    const int c = own[q] ? c3 + sz * (z0 + q) + ct + stt : 0;
-                    xq[q] = ld_stream(x + c);
-                    rq[q] = ld_stream(r + c);
-                }
+                for (int q = 0; q < LT_Z; ++q)
+                    rq[q] = ld_stream(r + (own[q] ? c3 + sz * (z0 + q) + ct + stt : 0));
             }
             if (t + 1 < t1) {
                 load_halo(t + 1);
@@ -1191,12 +1207,6 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
                     dd_add4(acc, pc[q], qc);
                 } else {
                     const int c = c3 + sz * k + ct;
-                    float4 xc = xo[q];
-                    xc.x = (float)((double)xc.x + alpha * (double)pc[q].x);
-                    xc.y = (float)((double)xc.y + alpha * (double)pc[q].y);
-                    xc.z = (float)((double)xc.z + alpha * (double)pc[q].z);
-                    xc.w = (float)((double)xc.w + alpha * (double)pc[q].w);
-                    st_stream(x + c, xc);
                     float4 rc = ro[q];
                     rc.x = (float)((double)rc.x + nalpha * (double)qc.x);
                     rc.y = (float)((double)rc.y + nalpha * (double)qc.y);
@@ -1215,10 +1225,7 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
             __syncthreads();
             if (UPDATE && t + 1 < t1) {
 #pragma unroll
-                for (int q = 0; q < LT_Z; ++q) {
-                    xo[q] = xq[q];
-                    ro[q] = rq[q];
-                }
+                for (int q = 0; q < LT_Z; ++q) ro[q] = rq[q];
             }
 #pragma unroll
             for (int q = 0; q < LT_Z; ++q) {
@@ -1604,47 +1611,67 @@ struct LevelWork {
     float *pmax() const { return (float *)(part.as<DD>() + 3 * cap); }
 };
 
-// one PCG iteration (the three stencil kernels and the two finalizers);
-// pbuf holds p in the storage of PS; ka/ku/kd: timer kinds or -1
+// The two buffers of the search direction, in its storage: p_k lives in
+// buffer k % 2 (FP32: W.p and W.q; bfloat16: the two halves of W.q; W.q is
+// free during the fused iterations).
+static void direction_buffers(LevelWork &W, int pstore, int64_t n, void *pb[2])
+{
+    if (pstore == FLOF_PSTORE_BF16) {
+        pb[0] = W.q.p;
+        pb[1] = (char *)W.q.p + (size_t)n * 8;
+    } else {
+        pb[0] = W.p.p;
+        pb[1] = W.q.p;
+    }
+}
+
+// one PCG iteration k (the three stencil kernels and the two finalizers):
+// p_k in pcur, p_(k+1) into pnext, odd = k % 2; ka/ku/kd: timer kinds or -1
 template <class PS>
-static void enqueue_tm_iter(flof_ctx *ctx, LevelWork &W, void *pbuf, const float *i1,
-                            const LevelGeom &G, float dxf, CgState *st, int ka, int ku, int kd,
-                            bool finalizers)
+static void enqueue_tm_iter(flof_ctx *ctx, LevelWork &W, void *pcur, void *pnext, bool odd,
+                            const float *i1, const LevelGeom &G, float dxf, CgState *st,
+                            int ka, int ku, int kd, bool finalizers)
 {
     typedef typename PS::T PT;
     const int ns = W.nb_stencil();
     if (W.lds)
-        klaunch(ctx, ka, k_cg_stencil_lds<PS, false>, dim3(ns), dim3(256), (const PT *)pbuf, i1,
-                (float4 *)nullptr, (float4 *)nullptr, G, W.tl, dxf, (const CgState *)st,
-                W.pdd(), (float *)nullptr);
+        klaunch(ctx, ka, k_cg_stencil_lds<PS, false>, dim3(ns), dim3(256), (const PT *)pcur, i1,
+                (float4 *)nullptr, G, W.tl, dxf, (const CgState *)st, W.pdd(),
+                (float *)nullptr);
     else
-        klaunch(ctx, ka, k_cg_pq_tm<PS>, dim3(ns), dim3(256), (const PT *)pbuf, i1, G, W.tm,
+        klaunch(ctx, ka, k_cg_pq_tm<PS>, dim3(ns), dim3(256), (const PT *)pcur, i1, G, W.tm,
                 dxf, (const CgState *)st, W.pdd());
     if (finalizers)
         hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(), ns);
     if (W.lds)
-        klaunch(ctx, ku, k_cg_stencil_lds<PS, true>, dim3(ns), dim3(256), (const PT *)pbuf, i1,
-                W.r.as<float4>(), W.x.as<float4>(), G, W.tl, dxf, (const CgState *)st, W.pdd(),
-                W.pmax());
+        klaunch(ctx, ku, k_cg_stencil_lds<PS, true>, dim3(ns), dim3(256), (const PT *)pcur, i1,
+                W.r.as<float4>(), G, W.tl, dxf, (const CgState *)st, W.pdd(), W.pmax());
     else
         klaunch(ctx, ku, k_cg_update_tm<PS>, dim3(ns), dim3(256), W.r.as<float4>(),
-                W.x.as<float4>(), (const PT *)pbuf, i1, G, W.tm, dxf, (const CgState *)st,
-                W.pdd(), W.pmax());
+                (const PT *)pcur, i1, G, W.tm, dxf, (const CgState *)st, W.pdd(), W.pmax());
     if (finalizers)
         hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
                            W.pmax(), ns);
-    klaunch(ctx, kd, k_cg_dir_tm<PS>, dim3(W.nb_tm), dim3(256), (PT *)pbuf,
-            (const float4 *)W.r.as<float4>(), i1, G, W.tm, dxf, (const CgState *)st);
+    if (odd)
+        klaunch(ctx, kd, k_cg_dir_tm<PS, true>, dim3(W.nb_tm), dim3(256), (const PT *)pcur,
+                (PT *)pnext, W.x.as<float4>(), (const float4 *)W.r.as<float4>(), i1, G, W.tm,
+                dxf, (const CgState *)st);
+    else
+        klaunch(ctx, kd, k_cg_dir_tm<PS, false>, dim3(W.nb_tm), dim3(256), (const PT *)pcur,
+                (PT *)pnext, W.x.as<float4>(), (const float4 *)W.r.as<float4>(), i1, G, W.tm,
+                dxf, (const CgState *)st);
 }
 
-static void enqueue_tm_iter_any(flof_ctx *ctx, int pstore, LevelWork &W, void *pbuf,
-                                const float *i1, const LevelGeom &G, float dxf, CgState *st,
-                                int ka, int ku, int kd, bool finalizers)
+static void enqueue_tm_iter_any(flof_ctx *ctx, int pstore, LevelWork &W, void *pcur,
+                                void *pnext, bool odd, const float *i1, const LevelGeom &G,
+                                float dxf, CgState *st, int ka, int ku, int kd, bool finalizers)
 {
     if (pstore == FLOF_PSTORE_BF16)
-        enqueue_tm_iter<PStoreBF16>(ctx, W, pbuf, i1, G, dxf, st, ka, ku, kd, finalizers);
+        enqueue_tm_iter<PStoreBF16>(ctx, W, pcur, pnext, odd, i1, G, dxf, st, ka, ku, kd,
+                                    finalizers);
     else
-        enqueue_tm_iter<PStoreF32>(ctx, W, pbuf, i1, G, dxf, st, ka, ku, kd, finalizers);
+        enqueue_tm_iter<PStoreF32>(ctx, W, pcur, pnext, odd, i1, G, dxf, st, ka, ku, kd,
+                                   finalizers);
 }
 
 // Autotuner of the t-march decomposition: on the first solve of a grid
@@ -1655,7 +1682,7 @@ static void enqueue_tm_iter_any(flof_ctx *ctx, int pstore, LevelWork &W, void *p
 // workspace is written by the set-up kernel afterwards, so the result of the
 // solve does not depend on the tuning.
 static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const float *i1,
-                     const LevelGeom &G, float dxf, int pstore, void *pbuf)
+                     const LevelGeom &G, float dxf, int pstore, void *const pb[2])
 {
     const std::array<int, 5> key{D.X, D.Y, D.Z, D.T, tune_variant(ctx)};
     auto it = ctx->tm_tuned.find(key);
@@ -1673,7 +1700,8 @@ static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const float *i
     FLOF_HIP(hipEventCreate(&e1));
     CgState *st = stt.as<CgState>();
     auto iterate = [&]() {
-        enqueue_tm_iter_any(ctx, pstore, W, pbuf, i1, G, dxf, st, -1, -1, -1, false);
+        enqueue_tm_iter_any(ctx, pstore, W, pb[0], pb[1], false, i1, G, dxf, st, -1, -1, -1,
+                            false);
     };
     int best = ctx->tm_blocks_per_cu;
     float best_ms = 1e30f;
@@ -1734,14 +1762,16 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
                                 ctx->stream));
         FLOF_HIP(hipStreamSynchronize(ctx->stream));  // maxIt lives on the stack
     }
-    // search direction storage: W.p (FP32) or, reduced, W.q (the fused
-    // kernels need no q array; the warm start uses it before the pack)
+    // search direction storage of the fused kernels: two buffers (W.p and
+    // W.q, or two bfloat16 halves of W.q; the fused kernels need no q array,
+    // and the warm start uses W.q before the iterations)
     const int pstore = ctx->use_tmarch ? ctx->p_store : FLOF_PSTORE_F32;
-    void *pbuf = pstore == FLOF_PSTORE_F32 ? W.p.p : W.q.p;
+    void *pb[2];
+    direction_buffers(W, pstore, n, pb);
     if (ctx->use_tmarch) {
         int bpc = ctx->tm_blocks_per_cu;
         if (ctx->autotune && n >= (1 << 16))
-            bpc = tuned_bpc(ctx, D, W, i1, G, dxf, pstore, pbuf);
+            bpc = tuned_bpc(ctx, D, W, i1, G, dxf, pstore, pb);
         W.set_tm(ctx, D, bpc);
     }
     // with fewer than 3 cells in a dimension every row is an identity row and
@@ -1788,7 +1818,7 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
     if (pstore == FLOF_PSTORE_BF16) {
         hipLaunchKernelGGL(k_p_pack<PStoreBF16>, dim3(grid_blocks(ctx, n)), dim3(256), 0,
                            ctx->stream, (const float4 *)W.p.as<float4>(),
-                           (PStoreBF16::T *)pbuf, n);
+                           (PStoreBF16::T *)pb[0], n);
         check_launch("k_p_pack");
     }
     // Iterations are enqueued in chunks of CHUNK (as one captured hipGraph
@@ -1796,8 +1826,10 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
     // pinned host memory behind an event, and the host waits only for the
     // chunk before the last one, so the GPU never idles on the check.
     // Kernels of iterations after convergence (or the cap, counted on the
-    // device) return at once.
+    // device) return at once.  CHUNK is even, so iteration k of a chunk has
+    // the parity of the solve's iteration and reads p from buffer k % 2.
     const int CHUNK = 16;
+    static_assert(CHUNK % 2 == 0, "direction buffers alternate per iteration");
     ctx->timer_tag = n;  // timed launches of this level carry its cell count
     bool timed = false;
     for (int k : {FLOF_KT_CG_APPLY, FLOF_KT_CG_UPDATE, FLOF_KT_CG_DIR})
@@ -1808,7 +1840,8 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
         const int kd = with_timers ? FLOF_KT_CG_DIR : -1;
         for (int k = 0; k < count; ++k) {
             if (ctx->use_tmarch) {
-                enqueue_tm_iter_any(ctx, pstore, W, pbuf, i1, G, dxf, st, ka, ku, kd, true);
+                enqueue_tm_iter_any(ctx, pstore, W, pb[k & 1], pb[(k + 1) & 1], (k & 1) != 0, i1,
+                                    G, dxf, st, ka, ku, kd, true);
                 continue;
             }
             klaunch(ctx, ka, k_cg_apply, dim3(nb), dim3(256),

@@ -8,7 +8,7 @@ import pytest
 from ofblend_amd import _lib
 from ofblend_amd import manta as M
 from ofblend_amd.abi import (FLOF_KT_CG_APPLY, FLOF_KT_CG_DIR, FLOF_KT_CG_UPDATE, FLOF_MIX_COPY,
-                             FLOF_MIX_DIR, FLOF_MIX_UPDATE)
+                             FLOF_MIX_DIR, FLOF_MIX_DIR_X, FLOF_MIX_UPDATE)
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -17,13 +17,13 @@ pytestmark = pytest.mark.gpu
 def test_stream_bandwidth_reports_and_rejects():
     L = _lib.lib()
     ctx = M.context()
-    for mix in (FLOF_MIX_UPDATE, FLOF_MIX_DIR, FLOF_MIX_COPY):
+    for mix in (FLOF_MIX_UPDATE, FLOF_MIX_DIR, FLOF_MIX_COPY, FLOF_MIX_DIR_X):
         g = C.c_double()
         assert L.flof_stream_bandwidth(ctx, mix, 1 << 22, 3, C.byref(g)) == 0
         print("mix", mix, "GB/s", g.value, flush=True)
         assert 100.0 < g.value < 8000.0
     g = C.c_double()
-    assert L.flof_stream_bandwidth(ctx, 3, 1 << 20, 3, C.byref(g)) != 0
+    assert L.flof_stream_bandwidth(ctx, 4, 1 << 20, 3, C.byref(g)) != 0
     assert L.flof_stream_bandwidth(ctx, FLOF_MIX_COPY, 0, 3, C.byref(g)) != 0
 
 
