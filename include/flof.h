@@ -214,6 +214,14 @@ enum flof_mix {
 };
 int flof_stream_bandwidth(flof_ctx *ctx, int mix, int64_t cells, int reps, double *gbps);
 
+/* Verification aid (no reference counterpart): compares the fast reciprocal
+ * the PCG kernels use for the Jacobi preconditioner (taken on a level only
+ * when every diagonal entry lies in [2^-126, 2^126)) with the IEEE division
+ * 1.f / d for every float whose bit pattern lies in [lo, hi] and stores the
+ * number of differing results (bitwise) in *mismatches.  The preconditioner
+ * must be the correctly rounded 1/d that the reference stores. */
+int flof_check_reciprocal(flof_ctx *ctx, uint32_t lo, uint32_t hi, uint64_t *mismatches);
+
 /* ---- Grid4d storage (source/grid4d.h:117-265, source/grid4d.cpp:60-138) - */
 /* Creates a zero-initialised X*Y*Z*T grid (x fastest, then y, z, t). */
 int flof_grid4d_create(flof_ctx *ctx, int X, int Y, int Z, int T, int elem,

@@ -1052,6 +1052,18 @@ k_stream_dirx(float4 *__restrict__ x, float4 *__restrict__ p, const float4 *__re
                                  rc.z * gc + b * pk.z, rc.w * gc + b * pk.w));
     }
 }
+// bitwise comparison of rcp_newton(d) with the IEEE division over bit patterns
+__global__ void k_check_rcp(uint32_t lo, uint32_t hi, unsigned long long *count)
+{
+    const uint64_t n = (uint64_t)hi - lo + 1;
+    unsigned long long bad = 0;
+    for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < n;
+         q += (uint64_t)gridDim.x * blockDim.x) {
+        const float d = __uint_as_float(lo + (uint32_t)q);
+        bad += __float_as_uint(rcp_newton(d)) != __float_as_uint(1.0f / d);
+    }
+    if (bad) atomicAdd(count, bad);
+}
 // copy: read a, write b (32 B per cell)
 __global__ void __launch_bounds__(256)
 k_stream_copy(float4 *__restrict__ b, const float4 *__restrict__ a, int64_t n)
@@ -1124,6 +1136,23 @@ int flof_stream_bandwidth(flof_ctx *ctx, int mix, int64_t cells, int reps, doubl
     FLOF_HIP(err);
     FLOF_HIP(hipGetLastError());
     *gbps = best;
+    API_CATCH(ctx)
+}
+
+int flof_check_reciprocal(flof_ctx *ctx, uint32_t lo, uint32_t hi, uint64_t *mismatches)
+{
+    API_TRY
+    FLOF_REQUIRE(ctx && mismatches && lo <= hi, FLOF_ERR_ARG,
+                 "flof_check_reciprocal: bad argument");
+    Tmp cnt(ctx, sizeof(unsigned long long));
+    FLOF_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), ctx->stream));
+    hipLaunchKernelGGL(k_check_rcp, dim3(ctx->num_cu * 8), dim3(256), 0, ctx->stream, lo, hi,
+                       cnt.as<unsigned long long>());
+    check_launch("k_check_rcp");
+    unsigned long long h = 0;
+    FLOF_HIP(hipMemcpyAsync(&h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    *mismatches = h;
     API_CATCH(ctx)
 }
 

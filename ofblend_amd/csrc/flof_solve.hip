@@ -73,7 +73,7 @@ struct CgState {
     int warm;    // warm start kept
     int zero;    // result must be zeroed (failed solve)
     int dir_mode;  // k_cg_dir: 0 skip, 1 x and p update, 2 x update only
-    int pad;
+    int fast_rcp;  // set-up: every interior g_d^2 + diag in rcp_newton's range
 };
 
 struct LevelGeom {
@@ -83,6 +83,8 @@ struct LevelGeom {
     float diag;   // 8*wSmooth + wEnergy, formed as the reference does
     int use_offd; // offd != 0
     float eps;    // VECTOR_EPSILON of FP32 builds
+    int fast_rcp; // every interior diagonal entry g_d^2 + diag of the level
+                  // lies in the range of rcp_newton (set after the set-up)
 };
 
 __device__ __forceinline__ bool interior(const Dims4 &D, int i, int j, int k, int t)
@@ -164,12 +166,13 @@ k_of_setup(const float *__restrict__ i0, const float *__restrict__ i1,
            float4 *__restrict__ g, float4 *__restrict__ prec,
            float4 *__restrict__ x, float4 *__restrict__ r, float4 *__restrict__ p,
            LevelGeom G, float wSmooth, float wEnergy, float mDt, float mDx,
-           float dxf, DD *part_dd, float *part_max)
+           float dxf, DD *part_dd, float *part_max, float *part_out)
 {
     const Dims4 &D = G.D;
     const int64_t n = (int64_t)D.st * D.T;
     DD sig{0., 0.};
     float mx = -INFINITY;
+    float out = 0.f;  // 1: a diagonal entry outside rcp_newton's range
     FOR_XCD(c, n)
     {
         int i, j, k, t;
@@ -213,6 +216,9 @@ k_of_setup(const float *__restrict__ i0, const float *__restrict__ i1,
         }
         const float4 rc = make_float4(b[0], b[1], b[2], b[3]);
         const float4 pr = prec_of(gc, G.diag);
+        if (!(rcp_exact_range(gc.x * gc.x + G.diag) && rcp_exact_range(gc.y * gc.y + G.diag) &&
+              rcp_exact_range(gc.z * gc.z + G.diag) && rcp_exact_range(gc.w * gc.w + G.diag)))
+            out = 1.f;
         const float4 z = f4mul(rc, pr);
         g[c] = gc;
         prec[c] = pr;
@@ -225,9 +231,11 @@ k_of_setup(const float *__restrict__ i0, const float *__restrict__ i1,
     }
     sig = block_dd(sig);
     mx = block_max(mx);
+    out = block_max(out);
     if (threadIdx.x == 0) {
         part_dd[blockIdx.x] = sig;
         part_max[blockIdx.x] = mx;
+        part_out[blockIdx.x] = out;
     }
 }
 
@@ -245,14 +253,18 @@ __device__ __forceinline__ void fin_dd_max(const DD *pd, const float *pm, int nb
     m = block_max(m);
 }
 
-__global__ void k_setup_fin(CgState *st, const DD *pd, const float *pm, int nb,
-                            float cgAcc, float eps)
+__global__ void k_setup_fin(CgState *st, const DD *pd, const float *pm, const float *pout,
+                            int nb, float cgAcc, float eps)
 {
     DD s;
     float m;
     fin_dd_max(pd, pm, nb, s, m);
+    float out = 0.f;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) out = fmaxf(out, pout[b]);
+    out = block_max(out);
     if (threadIdx.x == 0) {
         const double residual = (double)m;
+        st->fast_rcp = out == 0.f ? 1 : 0;
         st->iters = 0;
         st->warm = 0;
         st->zero = 0;
@@ -615,10 +627,16 @@ k_cg_dir(float4 *__restrict__ x, float4 *__restrict__ p, const float4 *__restric
 //                    also read x, previous p; write x 100 B  (odd iterations)
 // 148 B on average instead of 208 B of the three streaming kernels.
 
-__device__ __forceinline__ float4 prec_f32(const float4 &g, float diag)
+// the Jacobi preconditioner 1 / (g_d^2 + diag); fast (uniform over the
+// launch) when the set-up found every diagonal entry of the level in the
+// range where rcp_newton equals the division
+__device__ __forceinline__ float4 prec_f32(const float4 &g, const LevelGeom &G)
 {
-    return make_float4(1.f / (g.x * g.x + diag), 1.f / (g.y * g.y + diag),
-                       1.f / (g.z * g.z + diag), 1.f / (g.w * g.w + diag));
+    const float dx = g.x * g.x + G.diag, dy = g.y * g.y + G.diag;
+    const float dz = g.z * g.z + G.diag, dw = g.w * g.w + G.diag;
+    if (G.fast_rcp)
+        return make_float4(rcp_newton(dx), rcp_newton(dy), rcp_newton(dz), rcp_newton(dw));
+    return make_float4(1.f / dx, 1.f / dy, 1.f / dz, 1.f / dw);
 }
 
 // one thread's t-segment of one 3D cell
@@ -874,7 +892,7 @@ k_cg_update_tm(float4 *__restrict__ r, const typename PS::T *__restrict__ p,
             if (w.sp_int && t >= 1 && t < D.T - 1) {
                 const float4 gc = grad_i1(i1, c, D, im, ip, dxf);
                 qc = apply_A_tm<PS>(p, c, pm, pc, pp, gc, G);
-                pr = prec_f32(gc, G.diag);
+                pr = prec_f32(gc, G);
             } else {
                 qc = apply_I(pc);
                 pr = make_float4(1.f, 1.f, 1.f, 1.f);
@@ -947,7 +965,7 @@ k_cg_dir_tm(const typename PS::T *__restrict__ p, typename PS::T *pn, float4 *__
         if (mode == 1) {
             float4 pr = make_float4(1.f, 1.f, 1.f, 1.f);
             if (w.sp_int && t >= 1 && t < D.T - 1)
-                pr = prec_f32(grad_i1(i1, c, D, im, ip, dxf), G.diag);
+                pr = prec_f32(grad_i1(i1, c, D, im, ip, dxf), G);
             const float4 z = f4mul(ld_stream(r + c), pr);
             float4 np;
             np.x = (float)((double)z.x + beta * (double)pc.x);
@@ -1198,7 +1216,7 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
                         }
                     }
                     qc = make_float4(o[0], o[1], o[2], o[3]);
-                    if (UPDATE) pr = prec_f32(gc, G.diag);
+                    if (UPDATE) pr = prec_f32(gc, G);
                 } else {
                     qc = apply_I(pc[q]);
                     if (UPDATE) pr = make_float4(1.f, 1.f, 1.f, 1.f);
@@ -1756,6 +1774,7 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
     G.diag = diag;
     G.use_offd = G.offd != 0.f;
     G.eps = 1e-6f;
+    G.fast_rcp = 0;  // set from the set-up's range check below
     {
         int maxIt = maxIter;
         FLOF_HIP(hipMemcpyAsync(&st->maxIter, &maxIt, sizeof(int), hipMemcpyHostToDevice,
@@ -1780,10 +1799,10 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
     hipLaunchKernelGGL(k_of_setup, dim3(nb), dim3(256), 0, ctx->stream, i0, i1, vin, rhsT,
                        W.g.as<float4>(), W.prec.as<float4>(), W.x.as<float4>(),
                        W.r.as<float4>(), W.p.as<float4>(), G, wSmooth, wEnergy, dt, mDx, dxf,
-                       W.pdd(), W.pmax());
+                       W.pdd(), W.pmax(), (float *)W.pdd2());
     check_launch("k_of_setup");
     hipLaunchKernelGGL(k_setup_fin, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
-                       W.pmax(), nb, cgAccuracy, G.eps);
+                       W.pmax(), (const float *)W.pdd2(), nb, cgAccuracy, G.eps);
     check_launch("k_setup_fin");
     if (warm) {
         hipLaunchKernelGGL(k_warm_guess, dim3(nb), dim3(256), 0, ctx->stream, warm,
@@ -1859,6 +1878,12 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
                     (const CgState *)st);
         }
     };
+    // the state after the set-up (and warm start), which also says whether
+    // the fast reciprocal is exact on this level; G is captured by value
+    CgState hs;
+    FLOF_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    G.fast_rcp = hs.fast_rcp;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     if (!timed && ctx->use_graphs) {
@@ -1878,9 +1903,6 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
     hipEvent_t ev[2];
     FLOF_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     FLOF_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    CgState hs;
-    FLOF_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
-    FLOF_HIP(hipStreamSynchronize(ctx->stream));
     int64_t chunks = 0;
     const int64_t max_chunks = (int64_t)maxIter / CHUNK + 3;
     hipError_t loop_err = hipSuccess;

@@ -27,6 +27,25 @@ def test_stream_bandwidth_reports_and_rejects():
     assert L.flof_stream_bandwidth(ctx, FLOF_MIX_COPY, 0, 3, C.byref(g)) != 0
 
 
+def test_preconditioner_reciprocal_is_the_ieee_division():
+    """The fast reciprocal of the Jacobi preconditioner (hardware reciprocal
+    plus one Newton step) equals 1.f / d bit for bit for every float in
+    [2^-126, 2^126), the range the set-up requires of every diagonal entry
+    before the kernels use it, and differs outside it (denormals, d >= 2^126),
+    where the kernels keep the division."""
+    L = _lib.lib()
+    ctx = M.context()
+    bad = C.c_uint64(99)
+    assert L.flof_check_reciprocal(ctx, 0x00800000, 0x7e7fffff, C.byref(bad)) == 0
+    print("normal range: %d mismatches" % bad.value, flush=True)
+    assert bad.value == 0
+    for lo, hi in ((0x00000001, 0x007fffff), (0x7e800000, 0x7f7fffff)):
+        assert L.flof_check_reciprocal(ctx, lo, hi, C.byref(bad)) == 0
+        print("bits [%08x, %08x]: %d mismatches" % (lo, hi, bad.value), flush=True)
+        assert bad.value > 0
+    assert L.flof_check_reciprocal(ctx, 5, 4, C.byref(bad)) != 0
+
+
 def test_timing_tags_count_the_level_launches():
     """Every PCG iteration of a single-level solve is one launch of each of
     the three kinds, tagged with the level's cell count."""
