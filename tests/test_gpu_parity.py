@@ -421,6 +421,40 @@ def test_stencil_kernels_bit_exact(kind):
     assert np.array_equal(out["lds", FLOF_PSTORE_BF16][0], out["tmarch", FLOF_PSTORE_BF16][0])
 
 
+@pytest.mark.parametrize("cap", [1, 2, 5, 6])
+def test_cg_cap_at_both_iteration_parities(cap):
+    """The x update of an even PCG iteration is deferred to the next one: a
+    solve that stops at the iteration cap after an even or an odd iteration
+    (caps 1, 5 / 2, 6) must still hold every x update, in the LDS-tiled and
+    the t-marching form, bit for bit as the oracle with the same cap."""
+    from ofblend_amd import _lib
+    from ofblend_amd.abi import FLOF_STENCIL_AUTO, FLOF_STENCIL_LDS, FLOF_STENCIL_TMARCH
+    X, Y, Z, T = 70, 9, 10, 11
+    a = O.drop_sdf(X, Y, Z, T, seed=61)
+    b = O.drop_sdf(X, Y, Z, T, seed=62)
+    L = _lib.lib()
+    ctx = M.context()
+    s = solver(X, Y, Z, T)
+    ga, gb = real(s, a), real(s, b)
+    p = default_params(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5, cgMaxIter=cap)
+    zero = np.zeros((T, Z, Y, X, 4), np.float32)
+    with O.sum_mode(1):
+        ref, it, res, st = O.optical_flow4d(zero, a, b, p)
+    assert it == cap and st == 1
+    try:
+        for code in (FLOF_STENCIL_LDS, FLOF_STENCIL_TMARCH):
+            assert L.flof_ctx_set_stencil(ctx, code) == 0
+            vel = s.create(M.Grid4Vec4)
+            say("cap", cap, "stencil", code, (X, Y, Z, T))
+            M.opticalFlow4d(vel, ga, gb, wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5,
+                            cgMaxIter=cap)
+            info = M.getLastSolveInfo().summary()
+            assert info["levels"][0]["iters"] == [cap]
+            assert np.array_equal(vel.to_numpy(), ref)
+    finally:
+        L.flof_ctx_set_stencil(ctx, FLOF_STENCIL_AUTO)
+
+
 def test_stencil_kernels_degenerate_shapes():
     """The LDS-tiled kernels (forced) and the streaming q-array kernels on
     shapes with at most a few cells in some dimension, tiles sticking out on
