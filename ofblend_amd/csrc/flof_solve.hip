@@ -16,9 +16,11 @@
 // q = A p is bit-identical to the reference's applyMat.
 //
 // PCG per iteration: three stencil kernels (p.q, the r update with z.r and
-// max r, the x and direction update; 148 B per cell) with two 1-block FP64
-// finalizers between them (k_cg_alpha: alpha = sigma / p.q; k_cg_beta:
-// stopping test, beta).  p.q and the update run as LDS-tiled kernels
+// max r, the x and direction update; 148 B per cell).  The FP64 scalars:
+// alpha = sigma / p.q is reduced from the p.q partials by every workgroup of
+// the update itself (cg_alpha); the stopping test and beta by the 1-block
+// kernel k_cg_beta between the update and the direction kernel.  p.q and
+// the update run as LDS-tiled kernels
 // (k_cg_stencil_lds) on levels with long t segments and as t-marching
 // kernels (k_cg_pq_tm, k_cg_update_tm) elsewhere; the direction update is
 // k_cg_dir_tm.  The streaming variant with a q array (k_cg_apply,
@@ -251,6 +253,31 @@ __device__ __forceinline__ void fin_dd_max(const DD *pd, const float *pm, int nb
     }
     s = block_dd(s);
     m = block_max(m);
+}
+
+// alpha = sigma / p.q in the update kernels: every workgroup reduces the
+// p.q partials of the previous launch itself (the order of k_cg_alpha, so
+// all hold the same value and no finalizer launch runs between the two
+// kernels); workgroup 0 keeps alpha and the previous one in the state for
+// the direction kernel.  Called by all threads of the workgroup.
+__device__ __forceinline__ double cg_alpha(CgState *st, const DD *part_pq, int nb)
+{
+    DD s;
+    float m;
+    fin_dd_max(part_pq, nullptr, nb, s, m);
+    const double alpha = st->sigma / (s.hi + s.lo);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->alpha_prev = st->alpha;
+        st->alpha = alpha;
+    }
+    return alpha;
+}
+
+// iterations after the last one: the direction kernel does nothing (what
+// k_cg_alpha does for the streaming form)
+__device__ __forceinline__ void cg_after_last(CgState *st)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->dir_mode = 0;
 }
 
 __global__ void k_setup_fin(CgState *st, const DD *pd, const float *pm, const float *pout,
@@ -866,13 +893,16 @@ k_cg_pq_tm(const typename PS::T *__restrict__ p, const float *__restrict__ i1, L
 template <class PS>
 __global__ void __launch_bounds__(256)
 k_cg_update_tm(float4 *__restrict__ r, const typename PS::T *__restrict__ p,
-               const float *__restrict__ i1, LevelGeom G, TMarch M, float dxf,
-               const CgState *st, DD *part_dd, float *part_max)
+               const float *__restrict__ i1, LevelGeom G, TMarch M, float dxf, CgState *st,
+               const DD *part_pq, DD *part_dd, float *part_max)
 {
-    if (st->done) return;
+    if (st->done) {
+        cg_after_last(st);
+        return;
+    }
     const Dims4 &D = G.D;
     const TmCell w = tm_cell(D, M);
-    const double nalpha = -st->alpha;
+    const double nalpha = -cg_alpha(st, part_pq, (int)gridDim.x);
     DD sig{0., 0.};
     float mx = -INFINITY;
     if (w.live) {
@@ -1017,10 +1047,14 @@ __device__ __forceinline__ float4 f4scale(float s, const float4 &v)
 template <class PS, bool UPDATE>
 __global__ void __launch_bounds__(256)
 k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__ i1,
-                 float4 *__restrict__ r, LevelGeom G, TileMarch M,
-                 float dxf, const CgState *st, DD *part_dd, float *part_max)
+                 float4 *__restrict__ r, LevelGeom G, TileMarch M, float dxf, CgState *st,
+                 const DD *part_pq, DD *part_dd, float *part_max)
 {
-    if (st->done) return;  // uniform over the launch
+    if (st->done) {  // uniform over the launch
+        if (UPDATE) cg_after_last(st);
+        return;
+    }
+    double nalpha = 0.;
     // The neighbour sum of A p only ever uses offd * p(neighbour), so LDS
     // holds those products (each formed once per cell instead of once per
     // use; the same float products, so the sums are unchanged).
@@ -1085,7 +1119,6 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
             hx_ok = tid < 32 && hi >= 0 && hi < D.X && hj < D.Y && hk < D.Z;
             hx_c = hx_ok ? hi + D.X * hj + sz * hk : 0;
         }
-        const double nalpha = -st->alpha;
         const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
         // halo of one t step in registers until it is written to LDS
         float4 hy_v[2], hx_v, zm_v, zp_v;
@@ -1128,6 +1161,9 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
             for (int q = 0; q < LT_Z; ++q)
                 ro[q] = ld_stream(r + (own[q] ? c3 + sz * (z0 + q) + stt * t0 : 0));
         }
+        // alpha while the first step's loads are in flight (workgroup 0,
+        // which stores it, always has a work item)
+        if (UPDATE) nalpha = -cg_alpha(st, part_pq, (int)gridDim.x);
         for (int t = t0; t < t1; ++t) {
             const int ct = stt * t;
 #pragma unroll
@@ -1652,21 +1688,22 @@ static void enqueue_tm_iter(flof_ctx *ctx, LevelWork &W, void *pcur, void *pnext
 {
     typedef typename PS::T PT;
     const int ns = W.nb_stencil();
+    // p.q partials go to pdd2: the update reads them while its own
+    // workgroups write their partials to pdd
     if (W.lds)
         klaunch(ctx, ka, k_cg_stencil_lds<PS, false>, dim3(ns), dim3(256), (const PT *)pcur, i1,
-                (float4 *)nullptr, G, W.tl, dxf, (const CgState *)st, W.pdd(),
+                (float4 *)nullptr, G, W.tl, dxf, st, (const DD *)nullptr, W.pdd2(),
                 (float *)nullptr);
     else
         klaunch(ctx, ka, k_cg_pq_tm<PS>, dim3(ns), dim3(256), (const PT *)pcur, i1, G, W.tm,
-                dxf, (const CgState *)st, W.pdd());
-    if (finalizers)
-        hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(), ns);
+                dxf, (const CgState *)st, W.pdd2());
     if (W.lds)
         klaunch(ctx, ku, k_cg_stencil_lds<PS, true>, dim3(ns), dim3(256), (const PT *)pcur, i1,
-                W.r.as<float4>(), G, W.tl, dxf, (const CgState *)st, W.pdd(), W.pmax());
+                W.r.as<float4>(), G, W.tl, dxf, st, (const DD *)W.pdd2(), W.pdd(), W.pmax());
     else
         klaunch(ctx, ku, k_cg_update_tm<PS>, dim3(ns), dim3(256), W.r.as<float4>(),
-                (const PT *)pcur, i1, G, W.tm, dxf, (const CgState *)st, W.pdd(), W.pmax());
+                (const PT *)pcur, i1, G, W.tm, dxf, st, (const DD *)W.pdd2(), W.pdd(),
+                W.pmax());
     if (finalizers)
         hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
                            W.pmax(), ns);
