@@ -657,13 +657,17 @@ k_cg_dir(float4 *__restrict__ x, float4 *__restrict__ p, const float4 *__restric
 // the Jacobi preconditioner 1 / (g_d^2 + diag); fast (uniform over the
 // launch) when the set-up found every diagonal entry of the level in the
 // range where rcp_newton equals the division
-__device__ __forceinline__ float4 prec_f32(const float4 &g, const LevelGeom &G)
+__device__ __forceinline__ float4 prec_f32(const float4 &g, float diag, int fast)
 {
-    const float dx = g.x * g.x + G.diag, dy = g.y * g.y + G.diag;
-    const float dz = g.z * g.z + G.diag, dw = g.w * g.w + G.diag;
-    if (G.fast_rcp)
+    const float dx = g.x * g.x + diag, dy = g.y * g.y + diag;
+    const float dz = g.z * g.z + diag, dw = g.w * g.w + diag;
+    if (fast)
         return make_float4(rcp_newton(dx), rcp_newton(dy), rcp_newton(dz), rcp_newton(dw));
     return make_float4(1.f / dx, 1.f / dy, 1.f / dz, 1.f / dw);
+}
+__device__ __forceinline__ float4 prec_f32(const float4 &g, const LevelGeom &G)
+{
+    return prec_f32(g, G.diag, G.fast_rcp);
 }
 
 // one thread's t-segment of one 3D cell
@@ -1207,7 +1211,14 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
                 if (!own[q]) continue;
                 const int k = z0 + q;
                 float4 qc, pr;
-                if (sp_xy && k >= 1 && k < D.Z - 1 && t >= 1 && t < D.T - 1) {
+                // Branch-free: every lane evaluates the interior row.  On an
+                // outer-layer cell it runs with a neighbour sum of +0, g = 0
+                // and a diagonal of 1, which makes it the identity row
+                // ((((0 + 1 p_d) + 0 p_m ...), 1 * p_d = p_d) bit for bit and
+                // the preconditioner 1 / (0 + 1) = 1; the neighbour values it
+                // reads there lie in the tile's LDS slots and are discarded.
+                const bool inner = sp_xy && k >= 1 && k < D.Z - 1 && t >= 1 && t < D.T - 1;
+                {
                     float4 v = zero4;
                     if (G.use_offd) {
                         // offd * p of the neighbours in the order t-1, z-1, y-1,
@@ -1239,6 +1250,11 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
                         gc.z = (izq - izn) * dxf;
                     }
                     gc.w = (ip[q] - im[q]) * dxf;
+                    v = make_float4(inner ? v.x : 0.f, inner ? v.y : 0.f, inner ? v.z : 0.f,
+                                    inner ? v.w : 0.f);
+                    gc = make_float4(inner ? gc.x : 0.f, inner ? gc.y : 0.f,
+                                     inner ? gc.z : 0.f, inner ? gc.w : 0.f);
+                    const float dg = inner ? G.diag : 1.f;
                     const float gg[4] = {gc.x, gc.y, gc.z, gc.w};
                     const float pv[4] = {pc[q].x, pc[q].y, pc[q].z, pc[q].w};
                     float o[4] = {v.x, v.y, v.z, v.w};
@@ -1247,15 +1263,12 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
 #pragma unroll
                         for (int m = 0; m < 4; ++m) {
                             float b = gg[d] * gg[m];
-                            if (d == m) b = b + G.diag;
+                            if (d == m) b = b + dg;
                             o[d] = o[d] + b * pv[m];
                         }
                     }
                     qc = make_float4(o[0], o[1], o[2], o[3]);
-                    if (UPDATE) pr = prec_f32(gc, G);
-                } else {
-                    qc = apply_I(pc[q]);
-                    if (UPDATE) pr = make_float4(1.f, 1.f, 1.f, 1.f);
+                    if (UPDATE) pr = prec_f32(gc, dg, G.fast_rcp);
                 }
                 if (!UPDATE) {
                     dd_add4(acc, pc[q], qc);
