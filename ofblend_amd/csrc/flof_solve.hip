@@ -1018,8 +1018,10 @@ k_cg_dir_tm(const typename PS::T *__restrict__ p, typename PS::T *pn, float4 *__
 // A workgroup owns a tile of 64 (x) x 4 (y) x 4 (z) cells and marches it
 // through its t segment; thread (lane = x, wave = y) owns the column of 4
 // z cells.  Per t step the tile's p(t) and i1(t) go to LDS together with the
-// y-halo rows, x-halo columns and the z-halo planes; the t-1 / t+1 values of
-// the own cells stay in registers.  The halo of step t+1 and the own values
+// y-halo rows, x-halo columns and the z-halo planes, in one of two buffers
+// that alternate by step (so one barrier per step suffices, and the own
+// cells' t-1 products are read back from the other buffer); the t+1 values
+// of the own cells stay in registers.  The halo of step t+1 and the own values
 // of t+2 are loaded while step t computes (the loads are waited for only
 // when they are written to LDS at the next step).  Every cell is computed
 // with the same operands in the same order as in the untiled kernels, so the
@@ -1062,8 +1064,9 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
     // The neighbour sum of A p only ever uses offd * p(neighbour), so LDS
     // holds those products (each formed once per cell instead of once per
     // use; the same float products, so the sums are unchanged).
-    __shared__ float4 sp[LS_N];             // offd * p: tile, y-halo, x-halo
-    __shared__ float si[LS_N];              // i1 of the same cells
+    // double-buffered by t step: one barrier per step
+    __shared__ float4 sp2[2][LS_N];         // offd * p: tile, y-halo, x-halo
+    __shared__ float si2[2][LS_N];          // i1 of the same cells
     __shared__ float4 szp[2][LT_Y * LT_X];  // offd * p: z-halo planes z0-1, z0+4
     __shared__ float szi[2][LT_Y * LT_X];
     const Dims4 &D = G.D;
@@ -1142,13 +1145,13 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
             zp_i = i1[czp_l + ct];
         };
         // own cells: offd * p at t-1; p at t, t+1 and (prefetched) t+2
-        float4 pms[LT_Z], pc[LT_Z], pp[LT_Z], pn[LT_Z];
+        float4 pc[LT_Z], pp[LT_Z], pn[LT_Z];
         float im[LT_Z], ic[LT_Z], ip[LT_Z], in[LT_Z];
 #pragma unroll
         for (int q = 0; q < LT_Z; ++q) {
             const int c = c3 + sz * (z0 + q) + stt * t0;
             const bool hm = own[q] && t0 > 0, hp = own[q] && t0 + 1 < D.T;
-            pms[q] = f4scale(od, PS::ld(p, hm ? c - stt : 0));
+            sp2[1][lslot(q, wy, lane)] = f4scale(od, PS::ld(p, hm ? c - stt : 0));
             im[q] = i1[hm ? c - stt : 0];
             pc[q] = PS::ld(p, own[q] ? c : 0);
             ic[q] = i1[own[q] ? c : 0];
@@ -1170,6 +1173,13 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
         if (UPDATE) nalpha = -cg_alpha(st, part_pq, (int)gridDim.x);
         for (int t = t0; t < t1; ++t) {
             const int ct = stt * t;
+            // step t writes buffer (t - t0) & 1; the other one holds the own
+            // cells' offd * p(t-1) (written by this thread at the last step;
+            // before the loop for t0).  A buffer is written again two steps
+            // later, after every thread has passed the barrier between.
+            float4 *sp = sp2[(t - t0) & 1];
+            float *si = si2[(t - t0) & 1];
+            const float4 *spm = sp2[((t - t0) & 1) ^ 1];
 #pragma unroll
             for (int q = 0; q < LT_Z; ++q) {
                 sp[lslot(q, wy, lane)] = f4scale(od, pc[q]);
@@ -1224,7 +1234,7 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
                         // offd * p of the neighbours in the order t-1, z-1, y-1,
                         // x-1, x+1, y+1, z+1, t+1
                         const float4 nb[8] = {
-                            pms[q],
+                            spm[lslot(q, wy, lane)],
                             q > 0 ? sp[lslot(q > 0 ? q - 1 : 0, wy, lane)] : szp[0][wl],
                             sp[lslot(q, wy - 1, lane)],
                             sp[lslot(q, wy, lane - 1)],
@@ -1285,11 +1295,8 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
                     dd_add4(acc, z, rc);
                 }
             }
-            // offd * p(t) of the own cells is the t-1 neighbour of the next
-            // step (the thread's own LDS slots: no barrier needed)
-#pragma unroll
-            for (int q = 0; q < LT_Z; ++q) pms[q] = sp[lslot(q, wy, lane)];
-            __syncthreads();
+            // offd * p(t) of the own cells stays in this step's buffer as the
+            // t-1 neighbour of the next step (the thread's own slots)
             if (UPDATE && t + 1 < t1) {
 #pragma unroll
                 for (int q = 0; q < LT_Z; ++q) ro[q] = rq[q];
