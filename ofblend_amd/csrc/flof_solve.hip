@@ -1063,8 +1063,9 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
     double nalpha = 0.;
     // The neighbour sum of A p only ever uses offd * p(neighbour), so LDS
     // holds those products (each formed once per cell instead of once per
-    // use; the same float products, so the sums are unchanged).
-    // double-buffered by t step: one barrier per step
+    // use; the same float products, so the sums are unchanged).  The tile
+    // arrays are double-buffered by t step (one barrier per step); the
+    // z-halo planes need not be: each thread reads only the slots it wrote.
     __shared__ float4 sp2[2][LS_N];         // offd * p: tile, y-halo, x-halo
     __shared__ float si2[2][LS_N];          // i1 of the same cells
     __shared__ float4 szp[2][LT_Y * LT_X];  // offd * p: z-halo planes z0-1, z0+4
