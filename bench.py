@@ -85,9 +85,8 @@ def main():
                          "then the bench's 64-frame 128^3 warp; prints its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; every box here has one "
-              "GPU and the FlOF solve is not sharded (run with --gpus 1)",
-              file=sys.stderr)
+        print("bench.py: this build measures one GPU: the FlOF solve is not "
+              "sharded across GPUs (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     if args.no_graphs:

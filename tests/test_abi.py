@@ -52,3 +52,15 @@ def test_errors_are_reported_not_thrown():
     assert L.flof_grid4d_create(None, 4, 4, 4, 4, 1, 0, None) != 0
     assert L.flof_sync(None) != 0
     assert L.flof_last_error(None) == b"null context"
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with a message before
+    any GPU work (runs on the CPU)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr and p.stdout == ""
