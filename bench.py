@@ -226,7 +226,8 @@ def main():
                                "relative residual 1e-5",
                    "grid": [N, N, N, N], "levels": len(levels),
                    "iters_per_level": iters, "final_err": info["err_final"],
-                   "residual_per_level": [l["residual"][0] for l in levels]},
+                   "residual_per_level": [l["residual"][0] for l in levels],
+                   "residual_l2_per_level": [l["residual_l2"][0] for l in levels]},
         "roofline": roofline,
         "kernels_finest_level": kst0,
         "kernels_all_levels": kst,

@@ -110,6 +110,9 @@ typedef struct flof_solve_info {
                                                     cold start (or fallback) */
     int32_t pad_;
     int64_t total_iters;
+    double residual_l2[FLOF_MAX_LEVELS][FLOF_MAX_STEPS]; /* ||r||_2 / ||b||_2
+                                                    of the final residual (an
+                                                    addition; b: the rhs)    */
 } flof_solve_info;
 
 /* ---- context ----------------------------------------------------------- */

@@ -98,6 +98,7 @@ class SolveInfo(C.Structure):
         ("warm_used", C.c_int32 * FLOF_MAX_LEVELS),
         ("pad_", C.c_int32),
         ("total_iters", C.c_int64),
+        ("residual_l2", (C.c_double * FLOF_MAX_STEPS) * FLOF_MAX_LEVELS),
     ]
 
     def summary(self):
@@ -109,6 +110,7 @@ class SolveInfo(C.Structure):
                 "dims": list(self.dims[l]),
                 "iters": [self.iters[l][s] for s in range(ns)],
                 "residual": [self.residual[l][s] for s in range(ns)],
+                "residual_l2": [self.residual_l2[l][s] for s in range(ns)],
                 "status": [self.status[l][s] for s in range(ns)],
                 "err": self.err_level[l],
                 "warm_used": self.warm_used[l],

@@ -67,6 +67,7 @@ struct CgState {
     double resIni;   // signed max of the initial residual
     double resRel;   // (float)(residual / resIni), as ret_residual
     double r2, b2;   // warm-start residual check
+    double bb, rr;   // ||b||^2 of the rhs and ||r||^2 of the final residual
     double gamma;    // warm-start scale
     int iters;
     int done;
@@ -253,6 +254,31 @@ __device__ __forceinline__ void fin_dd_max(const DD *pd, const float *pm, int nb
     }
     s = block_dd(s);
     m = block_max(m);
+}
+
+// ||v||^2 of a Vec4 array (FP64 squares of the FP32 values, correctly
+// rounded sum): partials, then a 1-block kernel into *out
+__global__ void __launch_bounds__(256)
+k_sumsq(const float4 *__restrict__ v, int64_t n, DD *part)
+{
+    DD s{0., 0.};
+    FOR_XCD(c, n)
+    {
+        const float4 a = v[c];
+        dd_add(s, (double)a.x * (double)a.x);
+        dd_add(s, (double)a.y * (double)a.y);
+        dd_add(s, (double)a.z * (double)a.z);
+        dd_add(s, (double)a.w * (double)a.w);
+    }
+    s = block_dd(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ void k_sumsq_fin(const DD *part, int nb, double *out)
+{
+    DD s;
+    float m;
+    fin_dd_max(part, nullptr, nb, s, m);
+    if (threadIdx.x == 0) *out = s.hi + s.lo;
 }
 
 // alpha = sigma / p.q in the update kernels: every workgroup reduces the
@@ -1593,6 +1619,7 @@ void corr_vels_of(flof_ctx *ctx, float4 *dst, float4 *vel, const float *phiOrg,
 struct SolveOut {
     int iters = 0, status = 0, warm = 0;
     double resRel = 0.;
+    double resL2 = 0.;  // ||r||_2 / ||b||_2 at the end
 };
 
 // t-march decomposition: enough workgroups to fill the chip, t split into
@@ -1862,6 +1889,12 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
     hipLaunchKernelGGL(k_setup_fin, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
                        W.pmax(), (const float *)W.pdd2(), nb, cgAccuracy, G.eps);
     check_launch("k_setup_fin");
+    // ||b||^2 while r still holds the rhs (x = 0)
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, ctx->stream,
+                       (const float4 *)W.r.as<float4>(), n, W.pdd3());
+    hipLaunchKernelGGL(k_sumsq_fin, dim3(1), dim3(256), 0, ctx->stream, (const DD *)W.pdd3(),
+                       nb, &st->bb);
+    check_launch("k_sumsq");
     if (warm) {
         hipLaunchKernelGGL(k_warm_guess, dim3(nb), dim3(256), 0, ctx->stream, warm,
                            warm_base, W.q.as<float4>(), D, G.C, mDx);
@@ -1999,6 +2032,16 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
     so.status = hs.status;
     so.warm = hs.warm;
     so.resRel = hs.resRel;
+    // ||r||_2 / ||b||_2 of the final (recursive) residual
+    hipLaunchKernelGGL(k_sumsq, dim3(W.nb), dim3(256), 0, ctx->stream,
+                       (const float4 *)W.r.as<float4>(), n, W.pdd3());
+    hipLaunchKernelGGL(k_sumsq_fin, dim3(1), dim3(256), 0, ctx->stream, (const DD *)W.pdd3(),
+                       W.nb, &st->rr);
+    check_launch("k_sumsq");
+    FLOF_HIP(hipMemcpyAsync(&slots[0], st, sizeof(CgState), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    so.resL2 = slots[0].bb > 0. ? std::sqrt(slots[0].rr / slots[0].bb) : 0.;
     return so;
 }
 
@@ -2015,6 +2058,7 @@ static void record(flof_solve_info *info, int level, int step, const SolveOut &s
     if (!info || level >= FLOF_MAX_LEVELS || step >= FLOF_MAX_STEPS) return;
     info->iters[level][step] = so.iters;
     info->residual[level][step] = so.resRel;
+    info->residual_l2[level][step] = so.resL2;
     info->status[level][step] = so.status;
     info->warm_used[level] = so.warm;
     if (step + 1 > info->steps[level]) info->steps[level] = step + 1;

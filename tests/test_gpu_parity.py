@@ -174,6 +174,10 @@ def test_optical_flow4d_level_bit_exact():
         ref, it, res, st, rhs = O.optical_flow4d(vin, i0, i1, p, want_rhs=True)
     say("level solve iterations gpu", info["levels"][0]["iters"], "oracle", it)
     assert info["levels"][0]["iters"] == [it]
+    # ||r||_2 / ||b||_2 of the final residual: converged, so well below 1
+    l2 = info["levels"][0]["residual_l2"][0]
+    say("residual: signed-max ratio", info["levels"][0]["residual"][0], "l2 ratio", l2)
+    assert 0.0 < l2 < 1e-2
     assert np.array_equal(rhsT.to_numpy(), rhs)
     assert np.array_equal(vel.to_numpy(), ref)
 
