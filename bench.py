@@ -428,7 +428,8 @@ def committed_traffic(kernel):
 def warm_start(M, s, vel, i0, i1, v_plain, iters_plain, ms_plain):
     """The solve warm-started by the trained network (weights committed in
     ofblend_amd/data) against the plain solve: iterations, time and the
-    SDF-alignment error of both."""
+    SDF-alignment error of both (calcLsDiff, and the RMS of i0 warped by the
+    flow minus i1)."""
     from ofblend_amd import warmstart as WS
     if not os.path.exists(WS.DEFAULT_WEIGHTS):
         return None
@@ -447,15 +448,28 @@ def warm_start(M, s, vel, i0, i1, v_plain, iters_plain, ms_plain):
         t = time.perf_counter() - t0
     wi = M.getLastSolveInfo().summary()
     vw = vel.to_numpy()
+    h1 = i1.to_numpy()
+
+    def rms_sdf(flow):
+        # RMS of i0 warped by the flow (advect4d) minus i1, over all cells
+        w = s.create(M.Grid4Real)
+        w.copyFrom(i0)
+        M.advect4d(flow, w)
+        d = w.to_numpy().astype(np.float64) - h1
+        return float(np.sqrt(np.mean(d * d)))
+
+    rms_warm = rms_sdf(vel)
     vel.clear()
     e_plain = M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, cgMaxIter=FIXED_RESIDUAL_CAP,
                                         **PARAMS)
+    rms_plain = rms_sdf(vel)
     it_w = [l["iters"][0] for l in wi["levels"]]
     return {"ms_including_net": 1000.0 * t, "ms_plain": ms_plain,
             "iters_per_level": it_w, "iters_per_level_plain": iters_plain,
             "finest_level_iteration_ratio": iters_plain[0] / max(1, it_w[0]),
             "warm_kept_per_level": [l["warm_used"] for l in wi["levels"]],
             "final_err": err, "final_err_plain": e_plain,
+            "rms_sdf_diff": rms_warm, "rms_sdf_diff_plain": rms_plain,
             "max_abs_diff_vs_plain": float(np.abs(vw - v_plain).max()),
             "net_level": 32}
 
