@@ -83,6 +83,11 @@ def main():
                     help="short workload for rocprofv3 --pmc runs of the frame "
                          "warp: a 64^4 solve capped at 50 iterations per level, "
                          "then the bench's 64-frame 128^3 warp; prints its JSON")
+    ap.add_argument("--ops-workload", action="store_true",
+                    help="the path's other operations at 64^4 (grid ops, advect4d, "
+                         "interpolateGrid4d, gaussianBlur4d, calcLsDiff4d, "
+                         "corrVelsOf4d, preprocessing), each timed over repeated "
+                         "calls against its algorithmic bytes; prints its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU: the FlOF solve is not "
@@ -114,6 +119,11 @@ def main():
         M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, cgMaxIter=50, **PARAMS)
         M.sync()
         print(json.dumps(frame_warp(M, L, ctx, s, vel, FLOF_KT_WARP)), flush=True)
+        return
+    if args.ops_workload:
+        M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, **PARAMS)
+        M.sync()
+        print(json.dumps(ops_workload(M, s, i0, i1, vel)), flush=True)
         return
     if args.pmc_workload:
         # copy of a 64^4 Vec4 grid: 256 MiB read + 256 MiB written
@@ -510,6 +520,79 @@ def frame_warp(M, L, ctx, s, vel, kind):
             "frames_per_s": N / wall, "kernel_ms": 1000 * kern_s,
             "GBps_algorithmic": byts / kern_s / 1e9, "frac": byts / kern_s / HBM_PEAK,
             "place_ms": 1000 * t_place}
+
+
+def ops_workload(M, s, i0, i1, vel, reps=10):
+    """The path's operations other than the PCG and the frame warp, on the
+    64^4 bench grids (vel: the solved flow).  Each is called once to warm up,
+    then `reps` times back to back; the time per call is the wall time over
+    the calls, closed by a device synchronisation (calls that return a host
+    value synchronise every time, so theirs includes one round trip).
+    Algorithmic bytes: each cell's compulsory HBM reads and writes (gathers of
+    a neighbourhood count once)."""
+    cells = N ** 4
+    h = N // 2
+    sc = M.Solver(gridSize=M.vec3(h, h, h), fourthDim=h)
+    coarse_r = sc.create(M.Grid4Real)
+    coarse_v = sc.create(M.Grid4Vec4)
+    M.interpolateGrid4d(coarse_v, vel)
+    tmp_r = s.create(M.Grid4Real)
+    tmp_v = s.create(M.Grid4Vec4)
+    dst_v = s.create(M.Grid4Vec4)
+    phi = s.create(M.Grid4Real)
+
+    def advect():
+        tmp_r.copyFrom(i0)
+        M.advect4d(vel, tmp_r)
+
+    def prep():
+        phi.copyFrom(i1)
+        M.extrap4dLsSimple(phi, distance=4, inside=False)
+
+    ops = [
+        # name, call, algorithmic bytes per call, what the bytes are
+        ("grid4d copyFrom Vec4", lambda: tmp_v.copyFrom(vel), 32 * cells,
+         "read and write 16 B per cell"),
+        ("grid4d addScaled Real", lambda: tmp_r.addScaled(i1, 0.5), 12 * cells,
+         "read 2 x 4 B, write 4 B per cell"),
+        ("grid4d getMaxAbs Vec4", lambda: vel.getMaxAbs(), 16 * cells,
+         "read 16 B per cell (one host round trip)"),
+        ("advect4d Real by Vec4 (after a Real copy)", advect, (8 + 24) * cells,
+         "copy 8 B; advection reads vel 16 B and the source 4 B, writes 4 B per cell"),
+        ("interpolateGrid4d restrict Real 64^4 -> 32^4",
+         lambda: M.interpolateGrid4d(coarse_r, i0), 4 * cells + 4 * h ** 4,
+         "read the fine grid once, write the coarse one"),
+        ("interpolateGrid4d prolong Vec4 32^4 -> 64^4",
+         lambda: M.interpolateGrid4d(tmp_v, coarse_v), 16 * h ** 4 + 16 * cells,
+         "read the coarse grid once, write the fine one"),
+        ("gaussianBlur4d Vec4 sigma 1 (81 taps)",
+         lambda: M.gaussianBlur4d(tmp_v, sigma=1.0), 32 * cells,
+         "read and write 16 B per cell, taps from the caches"),
+        ("calcLsDiff4d", lambda: M.calcLsDiff4d(i0, i1), 8 * cells,
+         "read 2 x 4 B per cell (one host round trip)"),
+        ("corrVelsOf4d", lambda: M.corrVelsOf4d(dst_v, vel, i0, i0, i1), None,
+         "projection with its line searches; no single byte count"),
+        ("extrap4dLsSimple distance 4 (after a Real copy)", prep, None,
+         "layer-by-layer extrapolation; no single byte count"),
+        ("repeatFrame4d", lambda: M.repeatFrame4d(phi, 10.0), None,
+         "copies one t slice into the others"),
+    ]
+    out = {"grid": [N] * 4, "cells": cells, "reps": reps, "ops": []}
+    for name, fn, byts, what in ops:
+        log("bench: ops workload", name)
+        fn()
+        M.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        M.sync()
+        t = (time.perf_counter() - t0) / reps
+        rec = {"op": name, "ms_per_call": 1000 * t, "bytes": what}
+        if byts is not None:
+            rec["GBps_algorithmic"] = byts / t / 1e9
+            rec["frac"] = byts / t / HBM_PEAK
+        out["ops"].append(rec)
+    return out
 
 
 def _lib_check(rc, ctx):

@@ -1547,33 +1547,117 @@ __global__ void k_project_cells(float4 *__restrict__ dst, const float4 *__restri
 }
 
 // knCvExpolBlur4d (optflow4d.cpp:613-626): unmarked interior cells take the
-// 3^4 box mean, summed in the reference's loop order
-__global__ void k_fill_box81(const float4 *__restrict__ a, float4 *__restrict__ tmp,
-                             const float *__restrict__ marker, Dims4 D, Coord4 C)
+// 3^4 box mean, summed in the reference's loop order (t, z, y, x outermost
+// to innermost, FP32, starting from zero); other cells are copied.  A
+// workgroup owns a 64 x 2 x 2 column of cells for a segment of t and marches
+// through it, keeping the haloed planes t-1, t, t+1 (66 x 4 x 4 cells each)
+// in a ring of three LDS slots, so the 81 reads per cell hit LDS instead of
+// the vector caches.
+constexpr int BX_X = 64, BX_Y = 2, BX_Z = 2;
+constexpr int BX_HX = BX_X + 2, BX_HY = BX_Y + 2, BX_HZ = BX_Z + 2;
+constexpr int BX_PLANE = BX_HX * BX_HY * BX_HZ;  // 1056 cells
+constexpr int BX_LOADS = (BX_PLANE + 255) / 256;  // per thread and plane
+static_assert(BX_X * BX_Y * BX_Z == 256, "one cell per thread");
+
+struct BoxMarch {
+    int ntx, nty, ntz, ntile;  // column tiles
+    int seglen, nseg;          // t segments
+};
+
+__global__ void __launch_bounds__(256)
+k_fill_box81_lds(const float4 *__restrict__ a, float4 *__restrict__ out,
+                 const float *__restrict__ marker, Dims4 D, BoxMarch B)
 {
-    const int64_t n = (int64_t)D.st * D.T;
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        int i, j, k, t;
-        decompose((uint32_t)c, C, i, j, k, t);
-        if (!interior(D, i, j, k, t)) {
-            tmp[c] = a[c];
-            continue;
+    __shared__ float4 pl[3][BX_PLANE];
+    const int tile = blockIdx.x % B.ntile;
+    const int seg = blockIdx.x / B.ntile;
+    const int bx = tile % B.ntx, by = (tile / B.ntx) % B.nty, bz = tile / (B.ntx * B.nty);
+    const int x0 = bx * BX_X, y0 = by * BX_Y, z0 = bz * BX_Z;
+    const int t0 = seg * B.seglen;
+    const int t1 = min(t0 + B.seglen, D.T);
+    const int tx = threadIdx.x % BX_X, ty = (threadIdx.x / BX_X) % BX_Y,
+              tz = threadIdx.x / (BX_X * BX_Y);
+    const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+    const bool own = gx < D.X && gy < D.Y && gz < D.Z;
+
+    // halo cells this thread stages: e = threadIdx.x + 256 m
+    int64_t hoff[BX_LOADS];  // offset inside a t-plane, -1: outside the grid
+#pragma unroll
+    for (int m = 0; m < BX_LOADS; ++m) {
+        const int e = threadIdx.x + 256 * m;
+        hoff[m] = -1;
+        if (e < BX_PLANE) {
+            const int hx = e % BX_HX, hy = (e / BX_HX) % BX_HY, hz = e / (BX_HX * BX_HY);
+            const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+            if (x >= 0 && x < D.X && y >= 0 && y < D.Y && z >= 0 && z < D.Z)
+                hoff[m] = x + (int64_t)D.X * y + D.sz * z;
         }
-        if (marker[c] != 0.f) {
-            tmp[c] = a[c];
-            continue;
-        }
-        float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int vt = -1; vt <= 1; ++vt)
-            for (int zk = -1; zk <= 1; ++zk)
-                for (int yj = -1; yj <= 1; ++yj)
-                    for (int xi = -1; xi <= 1; ++xi)
-                        val = f4add(val, a[c + xi + (int64_t)D.X * yj + D.sz * zk + D.st * vt]);
-        const double f = 1. / 81.0;
-        tmp[c] = make_float4((float)(val.x * f), (float)(val.y * f), (float)(val.z * f),
-                             (float)(val.w * f));
     }
+    float4 st[BX_LOADS];
+    auto fetch = [&](int p) {
+#pragma unroll
+        for (int m = 0; m < BX_LOADS; ++m)
+            st[m] = (p >= 0 && p < D.T && hoff[m] >= 0) ? a[hoff[m] + D.st * p]
+                                                        : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    auto stash = [&](int p) {
+        float4 *s = pl[(p + 3) % 3];
+#pragma unroll
+        for (int m = 0; m < BX_LOADS; ++m) {
+            const int e = threadIdx.x + 256 * m;
+            if (e < BX_PLANE) s[e] = st[m];
+        }
+    };
+    // planes t0-1 and t0 (t0-1 may be -1: never read, see below)
+    fetch(t0 - 1);
+    stash(t0 - 1);
+    fetch(t0);
+    stash(t0);
+    const int cl = (tx + 1) + BX_HX * ((ty + 1) + BX_HY * (tz + 1));
+    for (int t = t0; t < t1; ++t) {
+        fetch(t + 1);
+        __syncthreads();  // every wave is done with plane t-2's slot
+        stash(t + 1);
+        __syncthreads();
+        if (own) {
+            const int64_t c = gx + (int64_t)D.X * gy + D.sz * gz + D.st * t;
+            // interior cells have all 81 neighbours in the grid, so every
+            // slot value they read was loaded (planes t-1..t+1 lie in [0, T))
+            if (!interior(D, gx, gy, gz, t) || marker[c] != 0.f) {
+                out[c] = pl[t % 3][cl];
+            } else {
+                float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int vt = -1; vt <= 1; ++vt) {
+                    const float4 *s = pl[(t + vt + 3) % 3];
+#pragma unroll
+                    for (int zk = -1; zk <= 1; ++zk)
+#pragma unroll
+                        for (int yj = -1; yj <= 1; ++yj)
+#pragma unroll
+                            for (int xi = -1; xi <= 1; ++xi)
+                                val = f4add(val, s[cl + xi + BX_HX * (yj + BX_HY * zk)]);
+                }
+                const double f = 1. / 81.0;
+                out[c] = make_float4((float)(val.x * f), (float)(val.y * f),
+                                     (float)(val.z * f), (float)(val.w * f));
+            }
+        }
+    }
+}
+
+static BoxMarch make_boxmarch(flof_ctx *ctx, const Dims4 &D)
+{
+    BoxMarch B;
+    B.ntx = (D.X + BX_X - 1) / BX_X;
+    B.nty = (D.Y + BX_Y - 1) / BX_Y;
+    B.ntz = (D.Z + BX_Z - 1) / BX_Z;
+    B.ntile = B.ntx * B.nty * B.ntz;
+    const int target = ctx->num_cu * 6;  // three 50 KB workgroups per CU, twice over
+    int nseg = (target + B.ntile - 1) / B.ntile;
+    nseg = std::max(1, std::min(nseg, D.T));
+    B.seglen = (D.T + nseg - 1) / nseg;
+    B.nseg = (D.T + B.seglen - 1) / B.seglen;
+    return B;
 }
 
 void corr_vels_of(flof_ctx *ctx, float4 *dst, float4 *vel, const float *phiOrg,
@@ -1595,13 +1679,19 @@ void corr_vels_of(flof_ctx *ctx, float4 *dst, float4 *vel, const float *phiOrg,
                            phiOrg, phiTarget, marker.as<float>(), D, C, threshPhi, maxIter,
                            dt);
         check_launch("k_project_cells");
+        // the passes alternate between dst and tmp (every pass writes every
+        // cell); one copy at the end if the last pass wrote tmp
+        float4 *cur = dst, *nxt = tmp.as<float4>();
+        const BoxMarch BM = make_boxmarch(ctx, D);
         for (int it = 0; it < doCvExtrapol; ++it) {
-            hipLaunchKernelGGL(k_fill_box81, dim3(nb), dim3(256), 0, ctx->stream, dst,
-                               tmp.as<float4>(), marker.as<float>(), D, C);
-            check_launch("k_fill_box81");
-            FLOF_HIP(hipMemcpyAsync(dst, tmp.p, (size_t)n * 16, hipMemcpyDeviceToDevice,
-                                    ctx->stream));
+            hipLaunchKernelGGL(k_fill_box81_lds, dim3(BM.ntile * BM.nseg), dim3(256), 0,
+                               ctx->stream, cur, nxt, marker.as<float>(), D, BM);
+            check_launch("k_fill_box81_lds");
+            std::swap(cur, nxt);
         }
+        if (cur != dst)
+            FLOF_HIP(hipMemcpyAsync(dst, cur, (size_t)n * 16, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
         if (postVelBlur > 1e-6f)
             gaussian_blur(ctx, dst, D, true, (float)(0.5 * postVelBlur), 1);
         const int resetBnd = int(resetBndWidth * (float)D.X) + 1;
