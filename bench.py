@@ -8,7 +8,7 @@ signed-max residual 1e-5 -- the iteration cap is raised above the
 reference's 1000 so the residual is the same at every level).  The JSON line
 reports solves/s of K solves on the production launch path (PCG iterations
 as captured HIP graphs), the HBM roofline of the dominant kernel (the PCG
-update) from the same K solves repeated with a HIP event pair on every PCG
+direction kernel) from the same K solves repeated with a HIP event pair on every PCG
 launch on the library's stream (events force plain launches, which costs a
 few percent of the solve, so they are kept out of the timed solves), and the
 CPU oracle timed on a bounded sample of the same workload on this host.
@@ -566,8 +566,9 @@ def ops_workload(M, s, i0, i1, vel, reps=10):
          lambda: M.interpolateGrid4d(tmp_v, coarse_v), 16 * h ** 4 + 16 * cells,
          "read the coarse grid once, write the fine one"),
         ("gaussianBlur4d Vec4 sigma 1 (81 taps)",
-         lambda: M.gaussianBlur4d(tmp_v, sigma=1.0), 32 * cells,
-         "read and write 16 B per cell, taps from the caches"),
+         lambda: M.gaussianBlur4d(tmp_v, sigma=1.0), 64 * cells,
+         "two passes (into a temporary and back), each reading and writing "
+         "16 B per cell"),
         ("calcLsDiff4d", lambda: M.calcLsDiff4d(i0, i1), 8 * cells,
          "read 2 x 4 B per cell (one host round trip)"),
         ("corrVelsOf4d", lambda: M.corrVelsOf4d(dst_v, vel, i0, i0, i1), None,

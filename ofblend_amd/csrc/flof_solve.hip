@@ -1373,6 +1373,162 @@ __global__ void k_of_finish(const float4 *__restrict__ x, float4 *__restrict__ o
 }
 
 // ---------------------------------------------------------------------------
+// Passes over 3^4 neighbourhoods with LDS-staged planes.  A workgroup owns a
+// 64 x 2 x 2 column of cells for a segment of t and marches through it,
+// keeping the haloed planes t-1, t, t+1 (66 x 4 x 4 cells each) in a ring of
+// three LDS slots, so the 81 reads per cell hit LDS instead of the vector
+// caches.  Interior cells have all 81 neighbours in the grid.
+//  MARCH_BOX81: knCvExpolBlur4d (optflow4d.cpp:613-626): unmarked interior
+//    cells take the 3^4 box mean, summed in the reference's loop order (t,
+//    z, y, x outermost to innermost, FP32, starting from zero); other cells
+//    are copied.  `aux` is the marker grid.
+//  MARCH_BLUR1: the Gaussian blur of k_blur4d for a window of s = 1 (never
+//    clipped for an interior cell), same order and operations; other cells
+//    are left unchanged.  `aux` is the weight table by squared distance.
+enum { MARCH_BOX81 = 0, MARCH_BLUR1 = 1 };
+constexpr int BX_X = 64, BX_Y = 2, BX_Z = 2;
+constexpr int BX_HX = BX_X + 2, BX_HY = BX_Y + 2, BX_HZ = BX_Z + 2;
+constexpr int BX_PLANE = BX_HX * BX_HY * BX_HZ;  // 1056 cells
+constexpr int BX_LOADS = (BX_PLANE + 255) / 256;  // per thread and plane
+static_assert(BX_X * BX_Y * BX_Z == 256, "one cell per thread");
+
+struct BoxMarch {
+    int ntx, nty, ntz, ntile;  // column tiles
+    int seglen, nseg;          // t segments
+};
+
+template <class T, int OP>
+__global__ void __launch_bounds__(256)
+k_march3_lds(const T *__restrict__ a, T *__restrict__ out, const float *__restrict__ aux,
+             Dims4 D, BoxMarch B, float eps)
+{
+    typedef VOps<T> O;
+    __shared__ T pl[3][BX_PLANE];
+    const int tile = blockIdx.x % B.ntile;
+    const int seg = blockIdx.x / B.ntile;
+    const int bx = tile % B.ntx, by = (tile / B.ntx) % B.nty, bz = tile / (B.ntx * B.nty);
+    const int x0 = bx * BX_X, y0 = by * BX_Y, z0 = bz * BX_Z;
+    const int t0 = seg * B.seglen;
+    const int t1 = min(t0 + B.seglen, D.T);
+    const int tx = threadIdx.x % BX_X, ty = (threadIdx.x / BX_X) % BX_Y,
+              tz = threadIdx.x / (BX_X * BX_Y);
+    const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+    const bool own = gx < D.X && gy < D.Y && gz < D.Z;
+
+    // halo cells this thread stages: e = threadIdx.x + 256 m
+    int64_t hoff[BX_LOADS];  // offset inside a t-plane, -1: outside the grid
+#pragma unroll
+    for (int m = 0; m < BX_LOADS; ++m) {
+        const int e = threadIdx.x + 256 * m;
+        hoff[m] = -1;
+        if (e < BX_PLANE) {
+            const int hx = e % BX_HX, hy = (e / BX_HX) % BX_HY, hz = e / (BX_HX * BX_HY);
+            const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+            if (x >= 0 && x < D.X && y >= 0 && y < D.Y && z >= 0 && z < D.Z)
+                hoff[m] = x + (int64_t)D.X * y + D.sz * z;
+        }
+    }
+    T st[BX_LOADS];
+    auto fetch = [&](int p) {
+#pragma unroll
+        for (int m = 0; m < BX_LOADS; ++m)
+            st[m] = (p >= 0 && p < D.T && hoff[m] >= 0) ? a[hoff[m] + D.st * p] : O::zero();
+    };
+    auto stash = [&](int p) {
+        T *s = pl[(p + 3) % 3];
+#pragma unroll
+        for (int m = 0; m < BX_LOADS; ++m) {
+            const int e = threadIdx.x + 256 * m;
+            if (e < BX_PLANE) s[e] = st[m];
+        }
+    };
+    // planes t0-1 and t0 (t0-1 may be -1: never read, see below)
+    fetch(t0 - 1);
+    stash(t0 - 1);
+    fetch(t0);
+    stash(t0);
+    const int cl = (tx + 1) + BX_HX * ((ty + 1) + BX_HY * (tz + 1));
+    for (int t = t0; t < t1; ++t) {
+        fetch(t + 1);
+        __syncthreads();  // every wave is done with plane t-2's slot
+        stash(t + 1);
+        __syncthreads();
+        if (!own) continue;
+        const int64_t c = gx + (int64_t)D.X * gy + D.sz * gz + D.st * t;
+        const bool in = interior(D, gx, gy, gz, t);
+        if (OP == MARCH_BOX81) {
+            if (!in || aux[c] != 0.f) {
+                out[c] = pl[t % 3][cl];
+                continue;
+            }
+            T val = O::zero();
+            for (int vt = -1; vt <= 1; ++vt) {
+                const T *s = pl[(t + vt + 3) % 3];
+#pragma unroll
+                for (int zk = -1; zk <= 1; ++zk)
+#pragma unroll
+                    for (int yj = -1; yj <= 1; ++yj)
+#pragma unroll
+                        for (int xi = -1; xi <= 1; ++xi)
+                            val = O::add(val, s[cl + xi + BX_HX * (yj + BX_HY * zk)]);
+            }
+            const double f = 1. / 81.0;
+            if constexpr (sizeof(T) == 16) {
+                const float4 v = *(const float4 *)&val;
+                ((float4 *)out)[c] = make_float4((float)(v.x * f), (float)(v.y * f),
+                                                 (float)(v.z * f), (float)(v.w * f));
+            } else {
+                ((float *)out)[c] = (float)(*(const float *)&val * f);
+            }
+        } else {
+            if (!in) continue;
+            T val = O::zero();
+            float weight = 0.f;
+            for (int vt = -1; vt <= 1; ++vt) {
+                const T *s = pl[(t + vt + 3) % 3];
+                // z not unrolled: 9 taps in flight keep the Vec4 form at
+                // 2 or more waves per SIMD
+#pragma unroll 1
+                for (int zk = -1; zk <= 1; ++zk)
+#pragma unroll
+                    for (int yj = -1; yj <= 1; ++yj)
+#pragma unroll
+                        for (int xi = -1; xi <= 1; ++xi) {
+                            const float w = aux[vt * vt + zk * zk + yj * yj + xi * xi];
+                            weight = weight + w;
+                            val = O::add(val, O::muls(s[cl + xi + BX_HX * (yj + BX_HY * zk)], w));
+                        }
+            }
+            if (weight > eps) {
+                if constexpr (sizeof(T) == 16) {
+                    const float4 v = *(const float4 *)&val;
+                    ((float4 *)out)[c] = make_float4(v.x / weight, v.y / weight, v.z / weight,
+                                                     v.w / weight);
+                } else {
+                    ((float *)out)[c] = *(const float *)&val / weight;
+                }
+            } else {
+                out[c] = pl[t % 3][cl];
+            }
+        }
+    }
+}
+
+static BoxMarch make_boxmarch(flof_ctx *ctx, const Dims4 &D)
+{
+    BoxMarch B;
+    B.ntx = (D.X + BX_X - 1) / BX_X;
+    B.nty = (D.Y + BX_Y - 1) / BX_Y;
+    B.ntz = (D.Z + BX_Z - 1) / BX_Z;
+    B.ntile = B.ntx * B.nty * B.ntz;
+    const int target = ctx->num_cu * 6;  // three 50 KB workgroups per CU, twice over
+    int nseg = (target + B.ntile - 1) / B.ntile;
+    nseg = std::max(1, std::min(nseg, D.T));
+    B.seglen = (D.T + nseg - 1) / nseg;
+    B.nseg = (D.T + B.seglen - 1) / B.seglen;
+    return B;
+}
+
 // Gaussian blur (optflow4d.cpp:127-174): every interior cell sums the
 // (2s+1)^4 window clipped to the grid in the reference's order (t, z, y, x
 // from low to high), the weights and the values in FP32, then divides by
@@ -1439,7 +1595,22 @@ void gaussian_blur(flof_ctx *ctx, void *a, const Dims4 &D, bool vec4, float sigm
     Coord4 C = make_coord4(D);
     const int nb = grid_blocks_xcd(ctx, n);
     const float eps = 1e-6f;  // VECTOR_EPSILON of FP32 builds
+    const BoxMarch BM = make_boxmarch(ctx, D);
     auto pass = [&](const void *src, void *dst) {
+        if (s == 1) {
+            // the LDS-tiled form of the same sums (identical results)
+            const dim3 grid(BM.ntile * BM.nseg);
+            if (vec4)
+                hipLaunchKernelGGL((k_march3_lds<float4, MARCH_BLUR1>), grid, dim3(256), 0,
+                                   ctx->stream, (const float4 *)src, (float4 *)dst,
+                                   (const float *)wdev.as<float>(), D, BM, eps);
+            else
+                hipLaunchKernelGGL((k_march3_lds<float, MARCH_BLUR1>), grid, dim3(256), 0,
+                                   ctx->stream, (const float *)src, (float *)dst,
+                                   (const float *)wdev.as<float>(), D, BM, eps);
+            check_launch("k_march3_lds<blur1>");
+            return;
+        }
         if (vec4)
             hipLaunchKernelGGL(k_blur4d<float4>, dim3(nb), dim3(256), 0, ctx->stream,
                                (const float4 *)src, (float4 *)dst, D, C, s,
@@ -1546,120 +1717,6 @@ __global__ void k_project_cells(float4 *__restrict__ dst, const float4 *__restri
     }
 }
 
-// knCvExpolBlur4d (optflow4d.cpp:613-626): unmarked interior cells take the
-// 3^4 box mean, summed in the reference's loop order (t, z, y, x outermost
-// to innermost, FP32, starting from zero); other cells are copied.  A
-// workgroup owns a 64 x 2 x 2 column of cells for a segment of t and marches
-// through it, keeping the haloed planes t-1, t, t+1 (66 x 4 x 4 cells each)
-// in a ring of three LDS slots, so the 81 reads per cell hit LDS instead of
-// the vector caches.
-constexpr int BX_X = 64, BX_Y = 2, BX_Z = 2;
-constexpr int BX_HX = BX_X + 2, BX_HY = BX_Y + 2, BX_HZ = BX_Z + 2;
-constexpr int BX_PLANE = BX_HX * BX_HY * BX_HZ;  // 1056 cells
-constexpr int BX_LOADS = (BX_PLANE + 255) / 256;  // per thread and plane
-static_assert(BX_X * BX_Y * BX_Z == 256, "one cell per thread");
-
-struct BoxMarch {
-    int ntx, nty, ntz, ntile;  // column tiles
-    int seglen, nseg;          // t segments
-};
-
-__global__ void __launch_bounds__(256)
-k_fill_box81_lds(const float4 *__restrict__ a, float4 *__restrict__ out,
-                 const float *__restrict__ marker, Dims4 D, BoxMarch B)
-{
-    __shared__ float4 pl[3][BX_PLANE];
-    const int tile = blockIdx.x % B.ntile;
-    const int seg = blockIdx.x / B.ntile;
-    const int bx = tile % B.ntx, by = (tile / B.ntx) % B.nty, bz = tile / (B.ntx * B.nty);
-    const int x0 = bx * BX_X, y0 = by * BX_Y, z0 = bz * BX_Z;
-    const int t0 = seg * B.seglen;
-    const int t1 = min(t0 + B.seglen, D.T);
-    const int tx = threadIdx.x % BX_X, ty = (threadIdx.x / BX_X) % BX_Y,
-              tz = threadIdx.x / (BX_X * BX_Y);
-    const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
-    const bool own = gx < D.X && gy < D.Y && gz < D.Z;
-
-    // halo cells this thread stages: e = threadIdx.x + 256 m
-    int64_t hoff[BX_LOADS];  // offset inside a t-plane, -1: outside the grid
-#pragma unroll
-    for (int m = 0; m < BX_LOADS; ++m) {
-        const int e = threadIdx.x + 256 * m;
-        hoff[m] = -1;
-        if (e < BX_PLANE) {
-            const int hx = e % BX_HX, hy = (e / BX_HX) % BX_HY, hz = e / (BX_HX * BX_HY);
-            const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
-            if (x >= 0 && x < D.X && y >= 0 && y < D.Y && z >= 0 && z < D.Z)
-                hoff[m] = x + (int64_t)D.X * y + D.sz * z;
-        }
-    }
-    float4 st[BX_LOADS];
-    auto fetch = [&](int p) {
-#pragma unroll
-        for (int m = 0; m < BX_LOADS; ++m)
-            st[m] = (p >= 0 && p < D.T && hoff[m] >= 0) ? a[hoff[m] + D.st * p]
-                                                        : make_float4(0.f, 0.f, 0.f, 0.f);
-    };
-    auto stash = [&](int p) {
-        float4 *s = pl[(p + 3) % 3];
-#pragma unroll
-        for (int m = 0; m < BX_LOADS; ++m) {
-            const int e = threadIdx.x + 256 * m;
-            if (e < BX_PLANE) s[e] = st[m];
-        }
-    };
-    // planes t0-1 and t0 (t0-1 may be -1: never read, see below)
-    fetch(t0 - 1);
-    stash(t0 - 1);
-    fetch(t0);
-    stash(t0);
-    const int cl = (tx + 1) + BX_HX * ((ty + 1) + BX_HY * (tz + 1));
-    for (int t = t0; t < t1; ++t) {
-        fetch(t + 1);
-        __syncthreads();  // every wave is done with plane t-2's slot
-        stash(t + 1);
-        __syncthreads();
-        if (own) {
-            const int64_t c = gx + (int64_t)D.X * gy + D.sz * gz + D.st * t;
-            // interior cells have all 81 neighbours in the grid, so every
-            // slot value they read was loaded (planes t-1..t+1 lie in [0, T))
-            if (!interior(D, gx, gy, gz, t) || marker[c] != 0.f) {
-                out[c] = pl[t % 3][cl];
-            } else {
-                float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
-                for (int vt = -1; vt <= 1; ++vt) {
-                    const float4 *s = pl[(t + vt + 3) % 3];
-#pragma unroll
-                    for (int zk = -1; zk <= 1; ++zk)
-#pragma unroll
-                        for (int yj = -1; yj <= 1; ++yj)
-#pragma unroll
-                            for (int xi = -1; xi <= 1; ++xi)
-                                val = f4add(val, s[cl + xi + BX_HX * (yj + BX_HY * zk)]);
-                }
-                const double f = 1. / 81.0;
-                out[c] = make_float4((float)(val.x * f), (float)(val.y * f),
-                                     (float)(val.z * f), (float)(val.w * f));
-            }
-        }
-    }
-}
-
-static BoxMarch make_boxmarch(flof_ctx *ctx, const Dims4 &D)
-{
-    BoxMarch B;
-    B.ntx = (D.X + BX_X - 1) / BX_X;
-    B.nty = (D.Y + BX_Y - 1) / BX_Y;
-    B.ntz = (D.Z + BX_Z - 1) / BX_Z;
-    B.ntile = B.ntx * B.nty * B.ntz;
-    const int target = ctx->num_cu * 6;  // three 50 KB workgroups per CU, twice over
-    int nseg = (target + B.ntile - 1) / B.ntile;
-    nseg = std::max(1, std::min(nseg, D.T));
-    B.seglen = (D.T + nseg - 1) / nseg;
-    B.nseg = (D.T + B.seglen - 1) / B.seglen;
-    return B;
-}
-
 void corr_vels_of(flof_ctx *ctx, float4 *dst, float4 *vel, const float *phiOrg,
                   const float *phiTarget, const Dims4 &D, float threshPhi,
                   float postVelBlur, float resetBndWidth, int maxIter, float dt)
@@ -1684,9 +1741,10 @@ void corr_vels_of(flof_ctx *ctx, float4 *dst, float4 *vel, const float *phiOrg,
         float4 *cur = dst, *nxt = tmp.as<float4>();
         const BoxMarch BM = make_boxmarch(ctx, D);
         for (int it = 0; it < doCvExtrapol; ++it) {
-            hipLaunchKernelGGL(k_fill_box81_lds, dim3(BM.ntile * BM.nseg), dim3(256), 0,
-                               ctx->stream, cur, nxt, marker.as<float>(), D, BM);
-            check_launch("k_fill_box81_lds");
+            hipLaunchKernelGGL((k_march3_lds<float4, MARCH_BOX81>), dim3(BM.ntile * BM.nseg),
+                               dim3(256), 0, ctx->stream, (const float4 *)cur, nxt,
+                               (const float *)marker.as<float>(), D, BM, 0.f);
+            check_launch("k_march3_lds<box81>");
             std::swap(cur, nxt);
         }
         if (cur != dst)
