@@ -549,6 +549,14 @@ def ops_workload(M, s, i0, i1, vel, reps=10):
         phi.copyFrom(i1)
         M.extrap4dLsSimple(phi, distance=4, inside=False)
 
+    # corrVelsOf4d subtracts its correction from vel in place, and its pass
+    # count follows max|vel|: every call starts from the solved flow
+    vel_c = s.create(M.Grid4Vec4)
+
+    def corr():
+        vel_c.copyFrom(vel)
+        M.corrVelsOf4d(dst_v, vel_c, i0, i0, i1)
+
     ops = [
         # name, call, algorithmic bytes per call, what the bytes are
         ("grid4d copyFrom Vec4", lambda: tmp_v.copyFrom(vel), 32 * cells,
@@ -571,8 +579,9 @@ def ops_workload(M, s, i0, i1, vel, reps=10):
          "16 B per cell"),
         ("calcLsDiff4d", lambda: M.calcLsDiff4d(i0, i1), 8 * cells,
          "read 2 x 4 B per cell (one host round trip)"),
-        ("corrVelsOf4d", lambda: M.corrVelsOf4d(dst_v, vel, i0, i0, i1), None,
-         "projection with its line searches; no single byte count"),
+        ("corrVelsOf4d (vel restored before each call)", corr, None,
+         "projection with its line searches, then int(max|vel|) + 4 "
+         "extrapolation passes; no single byte count"),
         ("extrap4dLsSimple distance 4 (after a Real copy)", prep, None,
          "layer-by-layer extrapolation; no single byte count"),
         ("repeatFrame4d", lambda: M.repeatFrame4d(phi, 10.0), None,
