@@ -79,14 +79,21 @@ typedef struct flof_of_params {
     int32_t cgMaxIter;     /* 1000   CG iteration cap (optflow4d.cpp:502)     */
     /* Extensions of this library (no reference counterpart): */
     int32_t warmStart;     /* 0      1: use *warm as CG start at each level   */
-    int32_t precision;     /* 0      storage of the CG search direction:
-                                     FLOF_PSTORE_F32 (the reference's
-                                     arithmetic) or FLOF_PSTORE_BF16 (x, r, g
-                                     and all arithmetic stay FP32, scalars
-                                     FP64)                                    */
+    int32_t precision;     /* 0      storage precisions, a combination of:
+                                     FLOF_PSTORE_BF16: the CG search
+                                     direction in bfloat16;
+                                     FLOF_PREC_IMG_BF16: the images (i0, i1,
+                                     their pyramid levels and warps) in
+                                     bfloat16 and the gradients of i1 rounded
+                                     to binary16 (the mixed-precision solve;
+                                     no projection, rhsT, network or
+                                     capture).  0 (FLOF_PSTORE_F32) is the
+                                     reference's FP32 arithmetic; stencil
+                                     arithmetic is FP32 and CG scalars FP64
+                                     in every form                            */
 } flof_of_params;
 
-enum flof_pstore { FLOF_PSTORE_F32 = 0, FLOF_PSTORE_BF16 = 1 };
+enum flof_pstore { FLOF_PSTORE_F32 = 0, FLOF_PSTORE_BF16 = 1, FLOF_PREC_IMG_BF16 = 2 };
 
 #define FLOF_MAX_LEVELS 16
 #define FLOF_MAX_STEPS 10
@@ -313,6 +320,27 @@ int flof_optical_flow_multiscale4d(flof_grid4d *vel, flof_grid4d *i0,
                                    const flof_of_params *p,
                                    flof_grid4d *warm, flof_solve_info *info,
                                    float *err);
+/* opticalFlowMultiscale4d with the learned warm start (no reference
+ * counterpart: the reference starts every level's CG from zero,
+ * optflow4d.cpp:284): on each of the `levels` finest pyramid levels (-1: all)
+ * that run one plain solve, the network `net` (of vel's context) predicts
+ * the level's increment from the level's pre-warped i0 and its i1, and the
+ * CG starts from that guess scaled energy-optimally, or from zero where the
+ * scaled guess has a larger residual norm than zero (the fallback).  The
+ * converged flow solves the same linear systems as a cold start. */
+int flof_optical_flow_multiscale4d_net(flof_grid4d *vel, flof_grid4d *i0, flof_grid4d *i1,
+                                       const flof_of_params *p, flof_warmnet *net,
+                                       int levels, flof_solve_info *info, float *err);
+/* opticalFlowMultiscale4d that also copies out the pre-warped i0 of pyramid
+ * level `level` (0: the finest) and the increment that level's solve adds to
+ * the flow (the training pairs of the warm-start network; no reference
+ * counterpart).  i0w_out (REAL) and inc_out (VEC4) have that level's size;
+ * multiStep must be 1; vel receives the same flow as without the copies. */
+int flof_optical_flow_multiscale4d_capture(flof_grid4d *vel, flof_grid4d *i0,
+                                           flof_grid4d *i1, const flof_of_params *p,
+                                           int level, flof_grid4d *i0w_out,
+                                           flof_grid4d *inc_out, flof_solve_info *info,
+                                           float *err);
 /* Batched opticalFlowMultiscale4d (no reference counterpart; the reference
  * runs its solves one after another): n independent solves, sizes may
  * differ, all with parameters *p (warmStart must be 0), run on
@@ -326,6 +354,22 @@ int flof_optical_flow_multiscale4d_batch(flof_ctx *ctx, int n,
                                          const flof_of_params *p,
                                          int concurrency,
                                          flof_solve_info *infos, float *errs);
+/* Fused batch of opticalFlowMultiscale4d (no reference counterpart; the
+ * reference solves one problem at a time, optflow4d.cpp:274-329 per level):
+ * n independent solves, sizes may differ, all with parameters *p, which
+ * must ask for one plain solve per level (multiStep 1, postVelBlur 0,
+ * resetBndWidth <= 0, no final projection, no warm start).  The pyramids
+ * are built first; then the levels of equal index of all problems are
+ * solved together, every PCG iteration as one launch per kernel kind that
+ * spans all of them (a work-item table maps each workgroup to its problem
+ * and its solver state; converged problems return at once), chunks of
+ * iterations captured as HIP graphs.  Runs on ctx's stream; each result is
+ * bit-identical to a flof_optical_flow_multiscale4d call.  The finest levels
+ * of all problems are resident at once (96 B of PCG workspace per cell). */
+int flof_optical_flow_multiscale4d_fused(flof_ctx *ctx, int n, flof_grid4d *const *vel,
+                                         flof_grid4d *const *i0, flof_grid4d *const *i1,
+                                         const flof_of_params *p, flof_solve_info *infos,
+                                         float *errs);
 /* corrVelsOf4d (optflow4d.cpp:651-802,2121-2130). */
 int flof_corr_vels_of4d(flof_grid4d *dst, flof_grid4d *vel,
                         flof_grid4d *phiOrg, flof_grid4d *phiCurr,
@@ -416,6 +460,15 @@ int flof_uni_read(const char *name, void *host, int X, int Y, int Z, int T,
 /* Grid4d::save / Grid4d::load (grid4d.cpp:122-138). */
 int flof_grid4d_save(flof_grid4d *g, const char *name);
 int flof_grid4d_load(flof_grid4d *g, const char *name);
+/* loadVelRescaled (scenes/ofHelpers.py:91-100): the VEC4 deformation file
+ * `fname` (M4T2) is read into a grid of the deformation solver's size
+ * defoSize (NULL: the file's size; a file of another size is an error, as in
+ * readGrid4dUni, fileio.cpp:936-951), resampled into vel by
+ * interpolateGrid4dVec(vel, ., offset=defoOffset, scale=defoScale)
+ * (grid4d.cpp:549-557) and multiplied by defoFactor (Grid4d::multConst). */
+int flof_load_vel_rescaled(flof_grid4d *vel, const char *fname, const int defoSize[4],
+                           const float defoOffset[4], const float defoScale[4],
+                           const float defoFactor[4]);
 
 /* ---- file-based frame functions (optflow4d.cpp:1464-1595,1871-2105) ----- */
 /* loadPlaceGrid4d(fname, ...): `fname` is the reference's snprintf pattern

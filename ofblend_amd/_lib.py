@@ -88,8 +88,15 @@ SIGNATURES = {
                              C.POINTER(SolveInfo)], _I),
     "flof_optical_flow_multiscale4d": ([_P, _P, _P, _P, C.POINTER(OfParams), _P,
                                         C.POINTER(SolveInfo), _FP], _I),
+    "flof_optical_flow_multiscale4d_net": ([_P, _P, _P, C.POINTER(OfParams), _P, _I,
+                                            C.POINTER(SolveInfo), _FP], _I),
+    "flof_optical_flow_multiscale4d_capture": ([_P, _P, _P, C.POINTER(OfParams), _I, _P, _P,
+                                                C.POINTER(SolveInfo), _FP], _I),
     "flof_optical_flow_multiscale4d_batch": ([_P, _I, C.POINTER(_P), C.POINTER(_P),
                                               C.POINTER(_P), C.POINTER(OfParams), _I,
+                                              C.POINTER(SolveInfo), _FP], _I),
+    "flof_optical_flow_multiscale4d_fused": ([_P, _I, C.POINTER(_P), C.POINTER(_P),
+                                              C.POINTER(_P), C.POINTER(OfParams),
                                               C.POINTER(SolveInfo), _FP], _I),
     "flof_corr_vels_of4d": ([_P, _P, _P, _P, _P, _F, _F, _F, _F, _I], _I),
     "flof_load_place_grid4d": ([_P, _I, _I, _I, _P, _FP, _FP, _I, _I, _I, _F,
@@ -114,6 +121,7 @@ SIGNATURES = {
     "flof_uni_read": ([C.c_char_p, _P, _I, _I, _I, _I, _I], _I),
     "flof_grid4d_save": ([_P, C.c_char_p], _I),
     "flof_grid4d_load": ([_P, C.c_char_p], _I),
+    "flof_load_vel_rescaled": ([_P, C.c_char_p, C.POINTER(C.c_int), _FP, _FP, _FP], _I),
     "flof_load_place_grid4d_uni": ([C.c_char_p, _P, _FP, _FP, _I, _I, _I, _F, _FP,
                                     _F, _I, _F, _I, _F, _F], _I),
     "flof_warp_state_open": ([_P, C.c_char_p, _I, _I, _F, C.POINTER(_P)], _I),

@@ -18,6 +18,7 @@ FLOF_MAX_STEPS = 10
 
 FLOF_PSTORE_F32 = 0
 FLOF_PSTORE_BF16 = 1
+FLOF_PREC_IMG_BF16 = 2  # bfloat16 images, binary16 gradients (mixed precision)
 
 FLOF_STENCIL_AUTO = 0
 FLOF_STENCIL_TMARCH = 1

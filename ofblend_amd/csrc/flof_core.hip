@@ -395,9 +395,9 @@ void launch_set_bound(flof_ctx *ctx, void *g, const Dims4 &D, bool vec4,
 // resampling: target(c) = interp4(source, c*fac + off) * post
 // (grid4d.cpp:531-547, grid4d.h:463-471)
 
-template <class T>
-__global__ void k_resample(T *__restrict__ dst, Dims4 Dd, Coord4 Cd,
-                           const T *__restrict__ src, Dims4 Ds, float4 fac,
+template <class T, class S = T>
+__global__ void k_resample(S *__restrict__ dst, Dims4 Dd, Coord4 Cd,
+                           const S *__restrict__ src, Dims4 Ds, float4 fac,
                            float4 off, float post, int use_post)
 {
     const int64_t n = (int64_t)Dd.st * Dd.T;
@@ -409,9 +409,9 @@ __global__ void k_resample(T *__restrict__ dst, Dims4 Dd, Coord4 Cd,
         const float py = (float)j * fac.y + off.y;
         const float pz = (float)k * fac.z + off.z;
         const float pt = (float)t * fac.w + off.w;
-        T v = interp4<T>(src, Ds, px, py, pz, pt);
+        T v = interp4<T, S>(src, Ds, px, py, pz, pt);
         if (use_post) v = VOps<T>::muls(v, post);
-        dst[c] = v;
+        dst[c] = Stor<T, S>::st(v);
     }
 }
 
@@ -462,12 +462,56 @@ void launch_pyramid(flof_ctx *ctx, void *dst, const Dims4 &Dd, const void *src,
     check_launch("k_pyramid");
 }
 
+// the pyramid restriction of a bfloat16 image (FP32 interpolation of the
+// stored values, rounded once when stored)
+void launch_pyramid_bf16(flof_ctx *ctx, void *dst, const Dims4 &Dd, const void *src,
+                         const Dims4 &Ds)
+{
+    FLOF_REQUIRE(Ds.X >= 2 && Ds.Y >= 2 && Ds.Z >= 2 && Ds.T >= 2, FLOF_ERR_DIM,
+                 "interpolation needs at least 2 cells per dimension");
+    const float fx = (float)Ds.X / (float)Dd.X, fy = (float)Ds.Y / (float)Dd.Y,
+                fz = (float)Ds.Z / (float)Dd.Z, ft = (float)Ds.T / (float)Dd.T;
+    float4 f = make_float4(fx, fy, fz, ft);
+    float4 o = make_float4((float)(fx * 0.5), (float)(fy * 0.5), (float)(fz * 0.5),
+                           (float)(ft * 0.5));
+    const int64_t n = (int64_t)Dd.st * Dd.T;
+    hipLaunchKernelGGL((k_resample<float, bf16_t>), dim3(grid_blocks(ctx, n)), dim3(256), 0,
+                       ctx->stream, (bf16_t *)dst, Dd, make_coord4(Dd), (const bf16_t *)src,
+                       Ds, f, o, 1.f, 0);
+    check_launch("k_pyramid (bfloat16)");
+}
+
+__global__ void k_to_bf16(const float *__restrict__ src, bf16_t *__restrict__ dst, int64_t n)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x)
+        dst[c] = Stor<float, bf16_t>::st(src[c]);
+}
+
+void launch_to_bf16(flof_ctx *ctx, void *dst, const float *src, int64_t n)
+{
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid_blocks(ctx, n)), dim3(256), 0, ctx->stream, src,
+                       (bf16_t *)dst, n);
+    check_launch("k_to_bf16");
+}
+
+void launch_neumann_bf16(flof_ctx *ctx, void *g, const Dims4 &D, int w)
+{
+    FLOF_REQUIRE(w >= 0 && D.X >= w + 2 && D.Y >= w + 2 && D.Z >= w + 2 &&
+                     D.T >= w + 2,
+                 FLOF_ERR_DIM, "setBoundNeumann: grid too small for width");
+    const int64_t n = (int64_t)D.st * D.T;
+    hipLaunchKernelGGL(k_neumann<bf16_t>, dim3(grid_blocks(ctx, n)), dim3(256), 0,
+                       ctx->stream, (bf16_t *)g, D, make_coord4(D), w);
+    check_launch("k_neumann (bfloat16)");
+}
+
 // ---------------------------------------------------------------------------
 // semi-Lagrangian advection (optflow4d.cpp:1275-1290): interior cells sample
 // the source at c + 1/2 - (vel*vel_scale)*dt, the outer layer becomes 0.
 
-template <class T>
-__global__ void k_advect(T *__restrict__ dst, const T *__restrict__ src,
+template <class T, class S = T>
+__global__ void k_advect(S *__restrict__ dst, const S *__restrict__ src,
                          const float4 *__restrict__ vel, float4 vscale,
                          int use_vscale, Dims4 D, Coord4 C, float dt)
 {
@@ -478,7 +522,7 @@ __global__ void k_advect(T *__restrict__ dst, const T *__restrict__ src,
         decompose((uint32_t)c, C, i, j, k, t);
         if (i < 1 || j < 1 || k < 1 || t < 1 || i >= D.X - 1 || j >= D.Y - 1 ||
             k >= D.Z - 1 || t >= D.T - 1) {
-            dst[c] = VOps<T>::zero();
+            dst[c] = Stor<T, S>::st(VOps<T>::zero());
             continue;
         }
         float4 v = vel[c];
@@ -488,7 +532,7 @@ __global__ void k_advect(T *__restrict__ dst, const T *__restrict__ src,
         const float py = ((float)j + 0.5f) - v.y;
         const float pz = ((float)k + 0.5f) - v.z;
         const float pt = ((float)t + 0.5f) - v.w;
-        dst[c] = interp4<T>(src, D, px, py, pz, pt);
+        dst[c] = Stor<T, S>::st(interp4<T, S>(src, D, px, py, pz, pt));
     }
 }
 
@@ -538,11 +582,35 @@ void advect_cfl(flof_ctx *ctx, const float4 *vel, void *grid, const Dims4 &D,
     }
 }
 
+// the same for a bfloat16 image: every substep interpolates the stored
+// values in FP32 and rounds its result once when it stores it
+void advect_cfl_bf16(flof_ctx *ctx, const float4 *vel, void *grid, const Dims4 &D,
+                     float cfl, float dt)
+{
+    FLOF_REQUIRE(D.X >= 2 && D.Y >= 2 && D.Z >= 2 && D.T >= 2, FLOF_ERR_DIM,
+                 "advection needs at least 2 cells per dimension");
+    const int64_t n = (int64_t)D.st * D.T;
+    const float maxVel = host_max_norm_vec4(ctx, vel, n) * dt;
+    const int steps = int(maxVel / cfl) + 1;
+    const float sdt = dt / (float)steps;
+    const size_t bytes = (size_t)n * 2;
+    Tmp tmp(ctx, bytes);
+    Coord4 C = make_coord4(D);
+    const float4 one = make_float4(1.f, 1.f, 1.f, 1.f);
+    for (int s = 0; s < steps; ++s) {
+        klaunch(ctx, FLOF_KT_ADVECT, k_advect<float, bf16_t>, dim3(grid_blocks(ctx, n)),
+                dim3(256), tmp.as<bf16_t>(), (const bf16_t *)grid, vel, one, 0, D, C, sdt);
+        check_launch("k_advect (bfloat16)");
+        FLOF_HIP(hipMemcpyAsync(grid, tmp.p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // calcLsDiff (optflow4d.cpp:895-927): correctly rounded FP64 sum of
 // min(1, |a-b|*corr) over sign-mismatching cells of the bnd interior.
 
-__global__ void k_lsdiff(const float *__restrict__ a, const float *__restrict__ b,
+template <class S>
+__global__ void k_lsdiff(const S *__restrict__ a, const S *__restrict__ b,
                          float *__restrict__ out, Dims4 D, Coord4 C, float corr,
                          int bnd, DD *part)
 {
@@ -555,7 +623,7 @@ __global__ void k_lsdiff(const float *__restrict__ a, const float *__restrict__ 
         if (i < bnd || j < bnd || k < bnd || t < bnd || i >= D.X - bnd ||
             j >= D.Y - bnd || k >= D.Z - bnd || t >= D.T - bnd)
             continue;
-        const float va = a[c], vb = b[c];
+        const float va = Stor<float, S>::ld(a[c]), vb = Stor<float, S>::ld(b[c]);
         if ((va < 0.f && vb < 0.f) || (va >= 0.f && vb >= 0.f)) {
             if (out) out[c] = 0.f;
         } else {
@@ -584,7 +652,7 @@ static void enqueue_ls_sum(flof_ctx *ctx, const float *a, const float *b, float 
     const int64_t n = (int64_t)D.st * D.T;
     const int nb = grid_blocks_xcd(ctx, n);
     RedPtrs r = red_ptrs(ctx, (size_t)nb * sizeof(DD));
-    hipLaunchKernelGGL(k_lsdiff, dim3(nb), dim3(256), 0, ctx->stream, a, b, out, D,
+    hipLaunchKernelGGL(k_lsdiff<float>, dim3(nb), dim3(256), 0, ctx->stream, a, b, out, D,
                        make_coord4(D), corr, bnd, (DD *)r.part);
     check_launch("k_lsdiff");
     hipLaunchKernelGGL(k_dd_fin, dim3(1), dim3(256), 0, ctx->stream, (const DD *)r.part, nb,
@@ -600,6 +668,25 @@ static double ls_scale(double accu, const Dims4 &D, int bnd)
     if (D.T > 1) accu *= 1000.;
     accu *= 1. / double(sx * sy * sz * st);
     return accu;
+}
+
+double host_ls_diff_bf16(flof_ctx *ctx, const void *a, const void *b, const Dims4 &D,
+                         float corr, int bnd)
+{
+    const int64_t n = (int64_t)D.st * D.T;
+    const int nb = grid_blocks_xcd(ctx, n);
+    RedPtrs r = red_ptrs(ctx, (size_t)nb * sizeof(DD));
+    hipLaunchKernelGGL(k_lsdiff<bf16_t>, dim3(nb), dim3(256), 0, ctx->stream,
+                       (const bf16_t *)a, (const bf16_t *)b, nullptr, D, make_coord4(D), corr,
+                       bnd, (DD *)r.part);
+    check_launch("k_lsdiff (bfloat16)");
+    hipLaunchKernelGGL(k_dd_fin, dim3(1), dim3(256), 0, ctx->stream, (const DD *)r.part, nb,
+                       r.result);
+    check_launch("k_dd_fin");
+    double h;
+    FLOF_HIP(hipMemcpyAsync(&h, r.result, 8, hipMemcpyDeviceToHost, ctx->stream));
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    return (double)(float)ls_scale(h, D, bnd);
 }
 
 double host_ls_diff(flof_ctx *ctx, const float *a, const float *b, float *out,
@@ -754,6 +841,7 @@ __global__ void k_drops(float *out, int fx, int fy, int fz, int T, int Y, double
         out[c] = (float)v;
     }
 }
+
 
 }  // namespace flof
 

@@ -104,12 +104,15 @@ struct flof_ctx {
     // storage of the CG search direction for the call in progress
     // (FLOF_PSTORE_*), set from flof_of_params.precision by the entry points
     int p_store = FLOF_PSTORE_F32;
+    // images of the call in progress stored as bfloat16 with binary16
+    // gradients (the mixed-precision solve, flof_of_params.precision)
+    bool img_bf16 = false;
 };
 
 // the kernel variant part of the autotuner's key of a grid shape
 inline int tune_variant(const flof_ctx *ctx)
 {
-    return ctx->p_store + 2 * ctx->stencil;
+    return ctx->p_store + 2 * ctx->stencil + 16 * (ctx->img_bf16 ? 1 : 0);
 }
 
 struct flof_warp_state {
@@ -310,15 +313,60 @@ template <> struct VOps<float4> {
     __device__ static __forceinline__ float4 zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
 };
 
+// bfloat16 storage (round to nearest even on every store; NaN stays NaN)
+__device__ __forceinline__ float bf16_to_f32(unsigned short u)
+{
+    return __uint_as_float((unsigned)u << 16);
+}
+__device__ __forceinline__ unsigned short f32_to_bf16(float f)
+{
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+// a Real grid value stored as bfloat16 (the mixed-precision images)
+struct bf16_t {
+    unsigned short u;
+};
+// value of storage type S as arithmetic type T, and back (rounded)
+template <class T, class S> struct Stor {
+    __device__ static __forceinline__ T ld(const S &s) { return s; }
+    __device__ static __forceinline__ S st(const T &v) { return v; }
+};
+template <> struct Stor<float, bf16_t> {
+    __device__ static __forceinline__ float ld(const bf16_t &s) { return bf16_to_f32(s.u); }
+    __device__ static __forceinline__ bf16_t st(float v) { return bf16_t{f32_to_bf16(v)}; }
+};
+// x rounded to the nearest binary16 value (ties to even; subnormals kept;
+// magnitudes from 65520 on become infinite), returned as FP32: the FP16
+// gradients of the mixed-precision solve, by bit operations so the result
+// does not depend on the device's FP16 denormal mode
+__device__ __forceinline__ float round_f16(float x)
+{
+    const unsigned u = __float_as_uint(x);
+    const unsigned sign = u & 0x80000000u;
+    unsigned a = u & 0x7fffffffu;
+    if (a >= 0x7f800000u) return x;                               // inf, NaN
+    if (a >= 0x477ff000u) return __uint_as_float(sign | 0x7f800000u);  // >= 65520
+    if (a < 0x38800000u) {                                        // < 2^-14
+        const float q = rintf(fabsf(x) * 16777216.0f) * 5.9604644775390625e-8f;
+        return __uint_as_float(sign | __float_as_uint(q));
+    }
+    a += 0x0fffu + ((a >> 13) & 1u);
+    return __uint_as_float(sign | (a & ~0x1fffu));
+}
+
 // Quadrilinear sample of a cell-centred 4D grid at p (cells at c + 1/2).
 // Per dimension q = p - 1/2, i = (int)q (truncation), w1 = q - i; q < 0
 // clamps to (0; 1, 0), i >= n - 1 clamps to (n - 2; 0, 1); combination order
 // y, x, z, t -- util/vector4d.h:422-516.
-template <class T>
-__device__ __forceinline__ T interp4(const T *__restrict__ d, const Dims4 &D,
+template <class T, class S = T>
+__device__ __forceinline__ T interp4(const S *__restrict__ d, const Dims4 &D,
                                      float px, float py, float pz, float pt)
 {
     typedef VOps<T> O;
+    typedef Stor<T, S> L;
     px = px - 0.5f;
     py = py - 0.5f;
     pz = pz - 0.5f;
@@ -337,18 +385,18 @@ __device__ __forceinline__ T interp4(const T *__restrict__ d, const Dims4 &D,
     if (zi >= D.Z - 1) { zi = D.Z - 2; f0 = 0.f; f1 = 1.f; }
     if (ti >= D.T - 1) { ti = D.T - 2; g0 = 0.f; g1 = 1.f; }
     const int64_t sY = D.X, sZ = D.sz, sT = D.st;
-    const T *b = d + (int64_t)xi + sY * yi + sZ * zi + sT * ti;
+    const S *b = d + (int64_t)xi + sY * yi + sZ * zi + sT * ti;
     // y first, then x, z, t
-    T a00 = O::add(O::muls(b[0], t0), O::muls(b[sY], t1));
-    T a01 = O::add(O::muls(b[1], t0), O::muls(b[1 + sY], t1));
-    T a02 = O::add(O::muls(b[sZ], t0), O::muls(b[sY + sZ], t1));
-    T a03 = O::add(O::muls(b[1 + sZ], t0), O::muls(b[1 + sY + sZ], t1));
+    T a00 = O::add(O::muls(L::ld(b[0]), t0), O::muls(L::ld(b[sY]), t1));
+    T a01 = O::add(O::muls(L::ld(b[1]), t0), O::muls(L::ld(b[1 + sY]), t1));
+    T a02 = O::add(O::muls(L::ld(b[sZ]), t0), O::muls(L::ld(b[sY + sZ]), t1));
+    T a03 = O::add(O::muls(L::ld(b[1 + sZ]), t0), O::muls(L::ld(b[1 + sY + sZ]), t1));
     T z0 = O::add(O::muls(O::add(O::muls(a00, s0), O::muls(a01, s1)), f0),
                   O::muls(O::add(O::muls(a02, s0), O::muls(a03, s1)), f1));
-    T a10 = O::add(O::muls(b[sT], t0), O::muls(b[sT + sY], t1));
-    T a11 = O::add(O::muls(b[sT + 1], t0), O::muls(b[sT + 1 + sY], t1));
-    T a12 = O::add(O::muls(b[sT + sZ], t0), O::muls(b[sT + sY + sZ], t1));
-    T a13 = O::add(O::muls(b[sT + 1 + sZ], t0), O::muls(b[sT + 1 + sY + sZ], t1));
+    T a10 = O::add(O::muls(L::ld(b[sT]), t0), O::muls(L::ld(b[sT + sY]), t1));
+    T a11 = O::add(O::muls(L::ld(b[sT + 1]), t0), O::muls(L::ld(b[sT + 1 + sY]), t1));
+    T a12 = O::add(O::muls(L::ld(b[sT + sZ]), t0), O::muls(L::ld(b[sT + sY + sZ]), t1));
+    T a13 = O::add(O::muls(L::ld(b[sT + 1 + sZ]), t0), O::muls(L::ld(b[sT + 1 + sY + sZ]), t1));
     T z1 = O::add(O::muls(O::add(O::muls(a10, s0), O::muls(a11, s1)), f0),
                   O::muls(O::add(O::muls(a12, s0), O::muls(a13, s1)), f1));
     return O::add(O::muls(z0, g0), O::muls(z1, g1));
@@ -540,5 +588,18 @@ void launch_axpy4(flof_ctx *ctx, float4 *y, const float4 *x, float4 a,
                   int64_t n);
 void launch_scale4(flof_ctx *ctx, float4 *y, float4 a, int64_t n);
 void launch_fill(flof_ctx *ctx, void *p, size_t bytes);
+// bfloat16 images of the mixed-precision solve (Real grids, 2 B per cell)
+void launch_to_bf16(flof_ctx *ctx, void *dst, const float *src, int64_t n);
+void launch_pyramid_bf16(flof_ctx *ctx, void *dst, const Dims4 &Dd, const void *src,
+                         const Dims4 &Ds);
+void advect_cfl_bf16(flof_ctx *ctx, const float4 *vel, void *grid, const Dims4 &D,
+                     float cfl, float dt);
+void launch_neumann_bf16(flof_ctx *ctx, void *g, const Dims4 &D, int w);
+double host_ls_diff_bf16(flof_ctx *ctx, const void *a, const void *b, const Dims4 &D,
+                         float corr, int bnd);
+// the warm-start network's flow from (i0, i1) of one grid, on ctx's stream
+// (the network must belong to ctx); keeps no activations for a backward pass
+void warmnet_infer(flof_ctx *ctx, flof_warmnet *net, const float *i0, const float *i1,
+                   const Dims4 &D, float4 *flow);
 
 }  // namespace flof

@@ -507,6 +507,35 @@ int flof_warp_state_close(flof_warp_state *s)
     return FLOF_OK;
 }
 
+int flof_load_vel_rescaled(flof_grid4d *vel, const char *fname, const int defoSize[4],
+                           const float defoOffset[4], const float defoScale[4],
+                           const float defoFactor[4])
+{
+    flof_ctx *ctx = vel ? vel->ctx : nullptr;
+    API_TRY
+    FLOF_REQUIRE(vel && fname && defoOffset && defoScale && defoFactor, FLOF_ERR_ARG,
+                 "loadVelRescaled: null argument");
+    FLOF_REQUIRE(vel->elem == FLOF_VEC4, FLOF_ERR_TYPE, "loadVelRescaled: vel must be VEC4");
+    int d[4];
+    if (defoSize) {
+        for (int i = 0; i < 4; ++i) d[i] = defoSize[i];
+    } else {
+        uni_size(fname, d);
+    }
+    // the deformation grid of the defo solver's size; load() rejects a file
+    // of another size, as readGrid4dUni does
+    flof_grid4d *ipol = nullptr;
+    int rc = flof_grid4d_create(ctx, d[0], d[1], d[2], d[3], FLOF_VEC4, FLOF_F32, &ipol);
+    if (rc != FLOF_OK) return rc;
+    const float whole[4] = {-1.f, -1.f, -1.f, -1.f};
+    rc = flof_grid4d_load(ipol, fname);
+    if (rc == FLOF_OK) rc = flof_interpolate_grid4d(vel, ipol, defoOffset, defoScale, whole);
+    if (rc == FLOF_OK) rc = flof_grid4d_mult_const(vel, defoFactor);
+    flof_grid4d_destroy(ipol);  // waits for the stream before freeing
+    return rc;
+    API_CATCH(ctx)
+}
+
 int flof_grid4d_save(flof_grid4d *g, const char *name)
 {
     flof_ctx *ctx = g ? g->ctx : nullptr;

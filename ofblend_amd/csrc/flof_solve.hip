@@ -30,11 +30,14 @@
 // chunk; kernels of iterations past convergence return immediately.
 
 #include <algorithm>
+#include <functional>
+#include <memory>
 #include <array>
 #include <atomic>
 #include <thread>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "flof_internal.h"
@@ -162,9 +165,28 @@ __device__ __forceinline__ float4 prec_of(const float4 &g, float diag)
                        (float)(1.0 / (double)(g.w * g.w + diag)));
 }
 
+// Storage of the images i0 and i1 of a solve: FP32 (the reference's Real)
+// or, in the mixed-precision form, bfloat16 with every component of the
+// gradient of i1 rounded to binary16 where it is formed (FP32 arithmetic
+// otherwise, so a GPU and an oracle that round at the same points agree).
+struct ImgF32 {
+    typedef float T;
+    static __device__ __forceinline__ float ld(const float *p, int64_t c) { return p[c]; }
+    static __device__ __forceinline__ float gq(float g) { return g; }
+};
+struct ImgBF16 {
+    typedef bf16_t T;
+    static __device__ __forceinline__ float ld(const bf16_t *p, int64_t c)
+    {
+        return bf16_to_f32(p[c].u);
+    }
+    static __device__ __forceinline__ float gq(float g) { return round_f16(g); }
+};
+
 // --- set-up (optflow4d.cpp:398-493 + the start of solve :284-303) ---------
+template <class IM>
 __global__ void __launch_bounds__(256)
-k_of_setup(const float *__restrict__ i0, const float *__restrict__ i1,
+k_of_setup(const typename IM::T *__restrict__ i0, const typename IM::T *__restrict__ i1,
            const float4 *__restrict__ vin, float *__restrict__ rhsT,
            float4 *__restrict__ g, float4 *__restrict__ prec,
            float4 *__restrict__ x, float4 *__restrict__ r, float4 *__restrict__ p,
@@ -191,12 +213,12 @@ k_of_setup(const float *__restrict__ i0, const float *__restrict__ i1,
             mx = fmaxf(mx, 0.f);
             continue;
         }
-        const float tderiv = (i1[c] - i0[c]) / mDt;
+        const float tderiv = (IM::ld(i1, c) - IM::ld(i0, c)) / mDt;
         float4 gc;
-        gc.x = (i1[c + 1] - i1[c - 1]) * dxf;
-        gc.y = (i1[c + D.X] - i1[c - D.X]) * dxf;
-        gc.z = (i1[c + D.sz] - i1[c - D.sz]) * dxf;
-        gc.w = (i1[c + D.st] - i1[c - D.st]) * dxf;
+        gc.x = IM::gq((IM::ld(i1, c + 1) - IM::ld(i1, c - 1)) * dxf);
+        gc.y = IM::gq((IM::ld(i1, c + D.X) - IM::ld(i1, c - D.X)) * dxf);
+        gc.z = IM::gq((IM::ld(i1, c + D.sz) - IM::ld(i1, c - D.sz)) * dxf);
+        gc.w = IM::gq((IM::ld(i1, c + D.st) - IM::ld(i1, c - D.st)) * dxf);
         float b[4] = {-gc.x * tderiv, -gc.y * tderiv, -gc.z * tderiv, -gc.w * tderiv};
         if (vin) {
             const int nbx[8] = {0, 0, 0, -1, 1, 0, 0, 0};
@@ -286,13 +308,14 @@ __global__ void k_sumsq_fin(const DD *part, int nb, double *out)
 // all hold the same value and no finalizer launch runs between the two
 // kernels); workgroup 0 keeps alpha and the previous one in the state for
 // the direction kernel.  Called by all threads of the workgroup.
-__device__ __forceinline__ double cg_alpha(CgState *st, const DD *part_pq, int nb)
+__device__ __forceinline__ double cg_alpha(CgState *st, const DD *part_pq, int nb,
+                                           int bid = -1)
 {
     DD s;
     float m;
     fin_dd_max(part_pq, nullptr, nb, s, m);
     const double alpha = st->sigma / (s.hi + s.lo);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if ((bid < 0 ? (int)blockIdx.x : bid) == 0 && threadIdx.x == 0) {
         st->alpha_prev = st->alpha;
         st->alpha = alpha;
     }
@@ -301,9 +324,9 @@ __device__ __forceinline__ double cg_alpha(CgState *st, const DD *part_pq, int n
 
 // iterations after the last one: the direction kernel does nothing (what
 // k_cg_alpha does for the streaming form)
-__device__ __forceinline__ void cg_after_last(CgState *st)
+__device__ __forceinline__ void cg_after_last(CgState *st, int bid = -1)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->dir_mode = 0;
+    if ((bid < 0 ? (int)blockIdx.x : bid) == 0 && threadIdx.x == 0) st->dir_mode = 0;
 }
 
 __global__ void k_setup_fin(CgState *st, const DD *pd, const float *pm, const float *pout,
@@ -602,7 +625,8 @@ k_cg_update(float4 *__restrict__ r, const float4 *__restrict__ q,
 }
 
 // signed-max stopping test, then beta = z.r / sigma (optflow4d.cpp:311-325)
-__global__ void k_cg_beta(CgState *st, const DD *pd, const float *pm, int nb)
+__device__ __forceinline__ void cg_beta_body(CgState *st, const DD *pd, const float *pm,
+                                             int nb)
 {
     if (st->done) return;
     DD s;
@@ -635,6 +659,10 @@ __global__ void k_cg_beta(CgState *st, const DD *pd, const float *pm, int nb)
             }
         }
     }
+}
+__global__ void k_cg_beta(CgState *st, const DD *pd, const float *pm, int nb)
+{
+    cg_beta_body(st, pd, pm, nb);
 }
 
 __global__ void __launch_bounds__(256)
@@ -704,14 +732,14 @@ struct TmCell {
     bool live;
 };
 
-__device__ __forceinline__ TmCell tm_cell(const Dims4 &D, const TMarch &M)
+__device__ __forceinline__ TmCell tm_cell(const Dims4 &D, const TMarch &M, int bid, int nblk)
 {
     TmCell w;
-    const int per = gridDim.x / FLOF_NXCD;
+    const int per = nblk / FLOF_NXCD;
 #if FLOF_TM_XCD_MAP
-    const int L = (blockIdx.x % FLOF_NXCD) * per + blockIdx.x / FLOF_NXCD;
+    const int L = (bid % FLOF_NXCD) * per + bid / FLOF_NXCD;
 #else
-    const int L = blockIdx.x;
+    const int L = bid;
     (void)per;
 #endif
     w.live = false;
@@ -734,14 +762,15 @@ __device__ __forceinline__ TmCell tm_cell(const Dims4 &D, const TMarch &M)
 }
 
 // g at interior cell c; ipm / ipp are i1 at t-1 / t+1 (optflow4d.cpp:425-433)
-__device__ __forceinline__ float4 grad_i1(const float *__restrict__ i1, int64_t c,
+template <class IM>
+__device__ __forceinline__ float4 grad_i1(const typename IM::T *__restrict__ i1, int64_t c,
                                           const Dims4 &D, float ipm, float ipp, float dxf)
 {
     float4 gc;
-    gc.x = (i1[c + 1] - i1[c - 1]) * dxf;
-    gc.y = (i1[c + D.X] - i1[c - D.X]) * dxf;
-    gc.z = (i1[c + D.sz] - i1[c - D.sz]) * dxf;
-    gc.w = (ipp - ipm) * dxf;
+    gc.x = IM::gq((IM::ld(i1, c + 1) - IM::ld(i1, c - 1)) * dxf);
+    gc.y = IM::gq((IM::ld(i1, c + D.X) - IM::ld(i1, c - D.X)) * dxf);
+    gc.z = IM::gq((IM::ld(i1, c + D.sz) - IM::ld(i1, c - D.sz)) * dxf);
+    gc.w = IM::gq((ipp - ipm) * dxf);
     return gc;
 }
 
@@ -797,17 +826,6 @@ struct PStoreF32 {
 #endif
 };
 typedef unsigned short bf4v __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ float bf16_to_f32(unsigned short u)
-{
-    return __uint_as_float((unsigned)u << 16);
-}
-__device__ __forceinline__ unsigned short f32_to_bf16(float f)
-{
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);  // NaN
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
 struct PStoreBF16 {
     typedef bf4v T;
     // the value a store followed by a load gives
@@ -880,31 +898,32 @@ __device__ __forceinline__ float4 apply_A_tm(const typename PS::T *__restrict__ 
 }
 
 // p.(A p)
-template <class PS>
-__global__ void __launch_bounds__(256)
-k_cg_pq_tm(const typename PS::T *__restrict__ p, const float *__restrict__ i1, LevelGeom G,
-           TMarch M, float dxf, const CgState *st, DD *part_dd)
+template <class PS, class IM>
+__device__ __forceinline__ void
+cg_pq_tm_body(int bid, int nblk, const typename PS::T *__restrict__ p,
+              const typename IM::T *__restrict__ i1, const LevelGeom &G, const TMarch &M,
+              float dxf, const CgState *st, DD *part_dd)
 {
     if (st->done) return;
     const Dims4 &D = G.D;
-    const TmCell w = tm_cell(D, M);
+    const TmCell w = tm_cell(D, M, bid, nblk);
     DD acc{0., 0.};
     if (w.live) {
         const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
         const int64_t c3 = w.c3;
         float4 pm = w.t0 > 0 ? PS::ld(p, c3 + D.st * (w.t0 - 1)) : zero4;
         float4 pc = PS::ld(p, c3 + D.st * w.t0);
-        float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
-        float ic = i1[c3 + D.st * w.t0];
+        float im = w.t0 > 0 ? IM::ld(i1, c3 + D.st * (w.t0 - 1)) : 0.f;
+        float ic = IM::ld(i1, c3 + D.st * w.t0);
 #pragma unroll FLOF_TM_UNROLL
         for (int t = w.t0; t < w.t1; ++t) {
             const int64_t c = c3 + D.st * t;
             const bool tn = t + 1 < D.T;
             const float4 pp = tn ? PS::ld(p, c + D.st) : zero4;
-            const float ip = tn ? i1[c + D.st] : 0.f;
+            const float ip = tn ? IM::ld(i1, c + D.st) : 0.f;
             float4 qc;
             if (w.sp_int && t >= 1 && t < D.T - 1)
-                qc = apply_A_tm<PS>(p, c, pm, pc, pp, grad_i1(i1, c, D, im, ip, dxf), G);
+                qc = apply_A_tm<PS>(p, c, pm, pc, pp, grad_i1<IM>(i1, c, D, im, ip, dxf), G);
             else
                 qc = apply_I(pc);
             dd_add4(acc, pc, qc);
@@ -915,24 +934,32 @@ k_cg_pq_tm(const typename PS::T *__restrict__ p, const float *__restrict__ i1, L
         }
     }
     acc = block_dd(acc);
-    if (threadIdx.x == 0) part_dd[blockIdx.x] = acc;
+    if (threadIdx.x == 0) part_dd[bid] = acc;
+}
+template <class PS, class IM>
+__global__ void __launch_bounds__(256)
+k_cg_pq_tm(const typename PS::T *__restrict__ p, const typename IM::T *__restrict__ i1,
+           LevelGeom G, TMarch M, float dxf, const CgState *st, DD *part_dd)
+{
+    cg_pq_tm_body<PS, IM>((int)blockIdx.x, (int)gridDim.x, p, i1, G, M, dxf, st, part_dd);
 }
 
 // r -= alpha (A p) (optflow4d.cpp:310), z = r diag^-1, z.r and the signed
 // max of r (:311-317); x += alpha p (:309) is applied by k_cg_dir_tm
-template <class PS>
-__global__ void __launch_bounds__(256)
-k_cg_update_tm(float4 *__restrict__ r, const typename PS::T *__restrict__ p,
-               const float *__restrict__ i1, LevelGeom G, TMarch M, float dxf, CgState *st,
-               const DD *part_pq, DD *part_dd, float *part_max)
+template <class PS, class IM>
+__device__ __forceinline__ void
+cg_update_tm_body(int bid, int nblk, float4 *__restrict__ r,
+                  const typename PS::T *__restrict__ p, const typename IM::T *__restrict__ i1,
+                  const LevelGeom &G, const TMarch &M, float dxf, CgState *st,
+                  const DD *part_pq, DD *part_dd, float *part_max)
 {
     if (st->done) {
-        cg_after_last(st);
+        cg_after_last(st, bid);
         return;
     }
     const Dims4 &D = G.D;
-    const TmCell w = tm_cell(D, M);
-    const double nalpha = -cg_alpha(st, part_pq, (int)gridDim.x);
+    const TmCell w = tm_cell(D, M, bid, nblk);
+    const double nalpha = -cg_alpha(st, part_pq, nblk, bid);
     DD sig{0., 0.};
     float mx = -INFINITY;
     if (w.live) {
@@ -940,17 +967,17 @@ k_cg_update_tm(float4 *__restrict__ r, const typename PS::T *__restrict__ p,
         const int64_t c3 = w.c3;
         float4 pm = w.t0 > 0 ? PS::ld(p, c3 + D.st * (w.t0 - 1)) : zero4;
         float4 pc = PS::ld(p, c3 + D.st * w.t0);
-        float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
-        float ic = i1[c3 + D.st * w.t0];
+        float im = w.t0 > 0 ? IM::ld(i1, c3 + D.st * (w.t0 - 1)) : 0.f;
+        float ic = IM::ld(i1, c3 + D.st * w.t0);
 #pragma unroll FLOF_TM_UNROLL
         for (int t = w.t0; t < w.t1; ++t) {
             const int64_t c = c3 + D.st * t;
             const bool tn = t + 1 < D.T;
             const float4 pp = tn ? PS::ld(p, c + D.st) : zero4;
-            const float ip = tn ? i1[c + D.st] : 0.f;
+            const float ip = tn ? IM::ld(i1, c + D.st) : 0.f;
             float4 qc, pr;
             if (w.sp_int && t >= 1 && t < D.T - 1) {
-                const float4 gc = grad_i1(i1, c, D, im, ip, dxf);
+                const float4 gc = grad_i1<IM>(i1, c, D, im, ip, dxf);
                 qc = apply_A_tm<PS>(p, c, pm, pc, pp, gc, G);
                 pr = prec_f32(gc, G);
             } else {
@@ -975,9 +1002,18 @@ k_cg_update_tm(float4 *__restrict__ r, const typename PS::T *__restrict__ p,
     sig = block_dd(sig);
     mx = block_max(mx);
     if (threadIdx.x == 0) {
-        part_dd[blockIdx.x] = sig;
-        part_max[blockIdx.x] = mx;
+        part_dd[bid] = sig;
+        part_max[bid] = mx;
     }
+}
+template <class PS, class IM>
+__global__ void __launch_bounds__(256)
+k_cg_update_tm(float4 *__restrict__ r, const typename PS::T *__restrict__ p,
+               const typename IM::T *__restrict__ i1, LevelGeom G, TMarch M, float dxf,
+               CgState *st, const DD *part_pq, DD *part_dd, float *part_max)
+{
+    cg_update_tm_body<PS, IM>((int)blockIdx.x, (int)gridDim.x, r, p, i1, G, M, dxf, st,
+                              part_pq, part_dd, part_max);
 }
 
 // x += alpha p (optflow4d.cpp:309), then p = z + beta p with z = r diag^-1
@@ -996,26 +1032,27 @@ __device__ __forceinline__ float4 x_plus(const float4 &x, double a, const float4
                        (float)((double)x.w + a * (double)p.w));
 }
 
-template <class PS, bool ODD>
-__global__ void __launch_bounds__(256)
-k_cg_dir_tm(const typename PS::T *__restrict__ p, typename PS::T *pn, float4 *__restrict__ x,
-            const float4 *__restrict__ r, const float *__restrict__ i1, LevelGeom G, TMarch M,
-            float dxf, const CgState *st)
+template <class PS, bool ODD, class IM>
+__device__ __forceinline__ void
+cg_dir_tm_body(int bid, int nblk, const typename PS::T *__restrict__ p, typename PS::T *pn,
+               float4 *__restrict__ x, const float4 *__restrict__ r,
+               const typename IM::T *__restrict__ i1, const LevelGeom &G, const TMarch &M,
+               float dxf, const CgState *st)
 {
     const int mode = st->dir_mode;
     if (mode != 1 && mode != 2) return;
     const bool xupd = ODD || mode == 2;
     const Dims4 &D = G.D;
-    const TmCell w = tm_cell(D, M);
+    const TmCell w = tm_cell(D, M, bid, nblk);
     if (!w.live) return;
     const double alpha = st->alpha, alpha_prev = st->alpha_prev, beta = st->beta;
     const int64_t c3 = w.c3;
-    float im = w.t0 > 0 ? i1[c3 + D.st * (w.t0 - 1)] : 0.f;
-    float ic = i1[c3 + D.st * w.t0];
+    float im = w.t0 > 0 ? IM::ld(i1, c3 + D.st * (w.t0 - 1)) : 0.f;
+    float ic = IM::ld(i1, c3 + D.st * w.t0);
 #pragma unroll FLOF_TM_UNROLL
     for (int t = w.t0; t < w.t1; ++t) {
         const int64_t c = c3 + D.st * t;
-        const float ip = t + 1 < D.T ? i1[c + D.st] : 0.f;
+        const float ip = t + 1 < D.T ? IM::ld(i1, c + D.st) : 0.f;
         const float4 pc = PS::ld_once(p, c);
         if (xupd) {
             float4 xc = ld_stream(x + c);
@@ -1025,7 +1062,7 @@ k_cg_dir_tm(const typename PS::T *__restrict__ p, typename PS::T *pn, float4 *__
         if (mode == 1) {
             float4 pr = make_float4(1.f, 1.f, 1.f, 1.f);
             if (w.sp_int && t >= 1 && t < D.T - 1)
-                pr = prec_f32(grad_i1(i1, c, D, im, ip, dxf), G);
+                pr = prec_f32(grad_i1<IM>(i1, c, D, im, ip, dxf), G);
             const float4 z = f4mul(ld_stream(r + c), pr);
             float4 np;
             np.x = (float)((double)z.x + beta * (double)pc.x);
@@ -1037,6 +1074,15 @@ k_cg_dir_tm(const typename PS::T *__restrict__ p, typename PS::T *pn, float4 *__
         im = ic;
         ic = ip;
     }
+}
+template <class PS, bool ODD, class IM>
+__global__ void __launch_bounds__(256)
+k_cg_dir_tm(const typename PS::T *__restrict__ p, typename PS::T *pn, float4 *__restrict__ x,
+            const float4 *__restrict__ r, const typename IM::T *__restrict__ i1, LevelGeom G,
+            TMarch M, float dxf, const CgState *st)
+{
+    cg_dir_tm_body<PS, ODD, IM>((int)blockIdx.x, (int)gridDim.x, p, pn, x, r, i1, G, M, dxf,
+                                st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1830,8 +1876,8 @@ struct LevelWork {
         const bool fits = D.st * (int64_t)(D.T + 2) < (1ll << 31);
         const bool pays = ctx->p_store == FLOF_PSTORE_F32 && tl.seglen >= 16 &&
                           10 * D.X >= 9 * LT_X * tl.ntx;
-        lds = fits && (ctx->stencil == FLOF_STENCIL_LDS ||
-                       (ctx->stencil == FLOF_STENCIL_AUTO && pays));
+        lds = fits && !ctx->img_bf16 &&
+              (ctx->stencil == FLOF_STENCIL_LDS || (ctx->stencil == FLOF_STENCIL_AUTO && pays));
     }
     bool lds = false;  // p.Ap and the update run as the LDS-tiled kernels
     int nb_stencil() const { return lds ? nb_lds : nb_tm; }
@@ -1877,52 +1923,69 @@ static void direction_buffers(LevelWork &W, int pstore, int64_t n, void *pb[2])
 
 // one PCG iteration k (the three stencil kernels and the two finalizers):
 // p_k in pcur, p_(k+1) into pnext, odd = k % 2; ka/ku/kd: timer kinds or -1
-template <class PS>
+template <class PS, class IM>
 static void enqueue_tm_iter(flof_ctx *ctx, LevelWork &W, void *pcur, void *pnext, bool odd,
-                            const float *i1, const LevelGeom &G, float dxf, CgState *st,
-                            int ka, int ku, int kd, bool finalizers)
+                            const typename IM::T *i1, const LevelGeom &G, float dxf,
+                            CgState *st, int ka, int ku, int kd, bool finalizers)
 {
     typedef typename PS::T PT;
     const int ns = W.nb_stencil();
     // p.q partials go to pdd2: the update reads them while its own
-    // workgroups write their partials to pdd
-    if (W.lds)
-        klaunch(ctx, ka, k_cg_stencil_lds<PS, false>, dim3(ns), dim3(256), (const PT *)pcur, i1,
-                (float4 *)nullptr, G, W.tl, dxf, st, (const DD *)nullptr, W.pdd2(),
-                (float *)nullptr);
-    else
-        klaunch(ctx, ka, k_cg_pq_tm<PS>, dim3(ns), dim3(256), (const PT *)pcur, i1, G, W.tm,
+    // workgroups write their partials to pdd (the LDS tiles stage FP32
+    // images only: set_tm never selects them for bfloat16 images)
+    bool lds_done = false;
+    if constexpr (std::is_same<IM, ImgF32>::value) {
+        if (W.lds) {
+            klaunch(ctx, ka, k_cg_stencil_lds<PS, false>, dim3(ns), dim3(256), (const PT *)pcur,
+                    i1, (float4 *)nullptr, G, W.tl, dxf, st, (const DD *)nullptr, W.pdd2(),
+                    (float *)nullptr);
+            klaunch(ctx, ku, k_cg_stencil_lds<PS, true>, dim3(ns), dim3(256), (const PT *)pcur,
+                    i1, W.r.as<float4>(), G, W.tl, dxf, st, (const DD *)W.pdd2(), W.pdd(),
+                    W.pmax());
+            lds_done = true;
+        }
+    }
+    if (!lds_done) {
+        klaunch(ctx, ka, k_cg_pq_tm<PS, IM>, dim3(ns), dim3(256), (const PT *)pcur, i1, G, W.tm,
                 dxf, (const CgState *)st, W.pdd2());
-    if (W.lds)
-        klaunch(ctx, ku, k_cg_stencil_lds<PS, true>, dim3(ns), dim3(256), (const PT *)pcur, i1,
-                W.r.as<float4>(), G, W.tl, dxf, st, (const DD *)W.pdd2(), W.pdd(), W.pmax());
-    else
-        klaunch(ctx, ku, k_cg_update_tm<PS>, dim3(ns), dim3(256), W.r.as<float4>(),
+        klaunch(ctx, ku, k_cg_update_tm<PS, IM>, dim3(ns), dim3(256), W.r.as<float4>(),
                 (const PT *)pcur, i1, G, W.tm, dxf, st, (const DD *)W.pdd2(), W.pdd(),
                 W.pmax());
+    }
     if (finalizers)
         hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
                            W.pmax(), ns);
     if (odd)
-        klaunch(ctx, kd, k_cg_dir_tm<PS, true>, dim3(W.nb_tm), dim3(256), (const PT *)pcur,
+        klaunch(ctx, kd, k_cg_dir_tm<PS, true, IM>, dim3(W.nb_tm), dim3(256), (const PT *)pcur,
                 (PT *)pnext, W.x.as<float4>(), (const float4 *)W.r.as<float4>(), i1, G, W.tm,
                 dxf, (const CgState *)st);
     else
-        klaunch(ctx, kd, k_cg_dir_tm<PS, false>, dim3(W.nb_tm), dim3(256), (const PT *)pcur,
+        klaunch(ctx, kd, k_cg_dir_tm<PS, false, IM>, dim3(W.nb_tm), dim3(256), (const PT *)pcur,
                 (PT *)pnext, W.x.as<float4>(), (const float4 *)W.r.as<float4>(), i1, G, W.tm,
                 dxf, (const CgState *)st);
 }
 
 static void enqueue_tm_iter_any(flof_ctx *ctx, int pstore, LevelWork &W, void *pcur,
-                                void *pnext, bool odd, const float *i1, const LevelGeom &G,
+                                void *pnext, bool odd, const void *i1, const LevelGeom &G,
                                 float dxf, CgState *st, int ka, int ku, int kd, bool finalizers)
 {
+    if (ctx->img_bf16) {
+        const bf16_t *ib = (const bf16_t *)i1;
+        if (pstore == FLOF_PSTORE_BF16)
+            enqueue_tm_iter<PStoreBF16, ImgBF16>(ctx, W, pcur, pnext, odd, ib, G, dxf, st, ka,
+                                                 ku, kd, finalizers);
+        else
+            enqueue_tm_iter<PStoreF32, ImgBF16>(ctx, W, pcur, pnext, odd, ib, G, dxf, st, ka,
+                                                ku, kd, finalizers);
+        return;
+    }
+    const float *ifl = (const float *)i1;
     if (pstore == FLOF_PSTORE_BF16)
-        enqueue_tm_iter<PStoreBF16>(ctx, W, pcur, pnext, odd, i1, G, dxf, st, ka, ku, kd,
-                                    finalizers);
+        enqueue_tm_iter<PStoreBF16, ImgF32>(ctx, W, pcur, pnext, odd, ifl, G, dxf, st, ka, ku,
+                                            kd, finalizers);
     else
-        enqueue_tm_iter<PStoreF32>(ctx, W, pcur, pnext, odd, i1, G, dxf, st, ka, ku, kd,
-                                   finalizers);
+        enqueue_tm_iter<PStoreF32, ImgF32>(ctx, W, pcur, pnext, odd, ifl, G, dxf, st, ka, ku,
+                                           kd, finalizers);
 }
 
 // Autotuner of the t-march decomposition: on the first solve of a grid
@@ -1932,10 +1995,11 @@ static void enqueue_tm_iter_any(flof_ctx *ctx, int pstore, LevelWork &W, void *p
 // run, and keeps the fastest per shape for the context's lifetime.  The
 // workspace is written by the set-up kernel afterwards, so the result of the
 // solve does not depend on the tuning.
-static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const float *i1,
-                     const LevelGeom &G, float dxf, int pstore, void *const pb[2])
+static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const void *i1,
+                     const LevelGeom &G, float dxf, int pstore, void *const pb[2],
+                     bool fused = false)
 {
-    const std::array<int, 5> key{D.X, D.Y, D.Z, D.T, tune_variant(ctx)};
+    const std::array<int, 5> key{D.X, D.Y, D.Z, D.T, tune_variant(ctx) + (fused ? 64 : 0)};
     auto it = ctx->tm_tuned.find(key);
     if (it != ctx->tm_tuned.end()) return it->second;
     CgState hs;
@@ -1959,6 +2023,7 @@ static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const float *i
     hipError_t err = hipSuccess;
     for (int bpc : TM_CANDIDATES) {
         W.set_tm(ctx, D, bpc);
+        if (fused) W.lds = false;
         if (W.nb_tm > W.cap || W.nb_lds > W.cap) continue;
         iterate();  // warm-up
         err = hipEventRecord(e0, ctx->stream);
@@ -1982,22 +2047,49 @@ static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const float *i
     return best;
 }
 
-// Solves one level.  vin: incoming flow for the rhs (may be null = zero).
-// The result x/h is written to (accumulate ? += : =) out.
-SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const float4 *vin,
-                     float *rhsT, const Dims4 &D, float wSmooth, float wEnergy,
-                     float cgAccuracy, float dt, int maxIter, const float4 *warm,
-                     const float4 *warm_base, float4 *out, bool accumulate)
+// One level's PCG: workspace, geometry, direction buffers and the solver
+// state after the set-up (read back to the host).
+struct LevelCg {
+    LevelWork W;
+    LevelGeom G;
+    Dims4 D;
+    int64_t n = 0;
+    float mDx = 0.f, dxf = 0.f;
+    int pstore = FLOF_PSTORE_F32;
+    void *pb[2] = {nullptr, nullptr};
+    const void *i1 = nullptr;
+    CgState hs;
+    LevelCg(flof_ctx *ctx, const Dims4 &D_) : W(ctx, D_), D(D_)
+    {
+        n = (int64_t)D_.st * D_.T;
+    }
+    CgState *st() { return W.st.as<CgState>(); }
+};
+
+// The set-up of one level (optflow4d.cpp:398-493, the start of the CG
+// :284-303, the optional warm start).  vin: incoming flow for the rhs (may be
+// null = zero).  fused: the level joins a batch whose PCG launches span
+// several problems (t-marching kernels only).
+static void level_setup(flof_ctx *ctx, LevelCg &L, const void *i0, const void *i1,
+                        const float4 *vin, float *rhsT, float wSmooth, float wEnergy,
+                        float cgAccuracy, float dt, int maxIter, const float4 *warm,
+                        const float4 *warm_base, bool fused)
 {
+    const Dims4 &D = L.D;
     FLOF_REQUIRE(D.X >= 1 && D.Y >= 1 && D.Z >= 1 && D.T >= 1, FLOF_ERR_DIM,
                  "opticalFlow4d: bad size");
-    const int64_t n = (int64_t)D.st * D.T;
+    const int64_t n = L.n;
     FLOF_REQUIRE(n < (1ll << 31), FLOF_ERR_DIM, "opticalFlow4d: grid too large");
-    LevelWork W(ctx, D);
+    FLOF_REQUIRE(!fused || ctx->use_tmarch, FLOF_ERR_ARG,
+                 "fused batch: needs the t-marching kernels");
+    LevelWork &W = L.W;
+    L.i1 = i1;
     CgState *st = W.st.as<CgState>();
     const float mDx = (float)(1. / D.X);
     const float dxf = (float)(1. / (2. * (double)mDx));
-    LevelGeom G;
+    L.mDx = mDx;
+    L.dxf = dxf;
+    LevelGeom &G = L.G;
     G.D = D;
     G.C = make_coord4(D);
     G.offd = -wSmooth * 1.f;
@@ -2018,21 +2110,31 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
     // W.q, or two bfloat16 halves of W.q; the fused kernels need no q array,
     // and the warm start uses W.q before the iterations)
     const int pstore = ctx->use_tmarch ? ctx->p_store : FLOF_PSTORE_F32;
-    void *pb[2];
+    L.pstore = pstore;
+    void **pb = L.pb;
     direction_buffers(W, pstore, n, pb);
     if (ctx->use_tmarch) {
         int bpc = ctx->tm_blocks_per_cu;
         if (ctx->autotune && n >= (1 << 16))
-            bpc = tuned_bpc(ctx, D, W, i1, G, dxf, pstore, pb);
+            bpc = tuned_bpc(ctx, D, W, i1, G, dxf, pstore, pb, fused);
         W.set_tm(ctx, D, bpc);
+        if (fused) W.lds = false;
     }
     // with fewer than 3 cells in a dimension every row is an identity row and
     // the neighbour offsets are never used
     const int nb = W.nb;
-    hipLaunchKernelGGL(k_of_setup, dim3(nb), dim3(256), 0, ctx->stream, i0, i1, vin, rhsT,
-                       W.g.as<float4>(), W.prec.as<float4>(), W.x.as<float4>(),
-                       W.r.as<float4>(), W.p.as<float4>(), G, wSmooth, wEnergy, dt, mDx, dxf,
-                       W.pdd(), W.pmax(), (float *)W.pdd2());
+    if (ctx->img_bf16)
+        hipLaunchKernelGGL(k_of_setup<ImgBF16>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const bf16_t *)i0, (const bf16_t *)i1, vin, rhsT, W.g.as<float4>(),
+                           W.prec.as<float4>(), W.x.as<float4>(), W.r.as<float4>(),
+                           W.p.as<float4>(), G, wSmooth, wEnergy, dt, mDx, dxf, W.pdd(),
+                           W.pmax(), (float *)W.pdd2());
+    else
+        hipLaunchKernelGGL(k_of_setup<ImgF32>, dim3(nb), dim3(256), 0, ctx->stream,
+                           (const float *)i0, (const float *)i1, vin, rhsT, W.g.as<float4>(),
+                           W.prec.as<float4>(), W.x.as<float4>(), W.r.as<float4>(),
+                           W.p.as<float4>(), G, wSmooth, wEnergy, dt, mDx, dxf, W.pdd(),
+                           W.pmax(), (float *)W.pdd2());
     check_launch("k_of_setup");
     hipLaunchKernelGGL(k_setup_fin, dim3(1), dim3(256), 0, ctx->stream, st, W.pdd(),
                        W.pmax(), (const float *)W.pdd2(), nb, cgAccuracy, G.eps);
@@ -2079,6 +2181,60 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
                            (PStoreBF16::T *)pb[0], n);
         check_launch("k_p_pack");
     }
+    // the state after the set-up (and warm start), which also says whether
+    // the fast reciprocal is exact on this level
+    FLOF_HIP(hipMemcpyAsync(&L.hs, st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    G.fast_rcp = L.hs.fast_rcp;
+}
+
+// vel (=) or (+=) x / h of a finished level, and ||r||_2 / ||b||_2
+static SolveOut level_finish(flof_ctx *ctx, LevelCg &L, float4 *out, bool accumulate)
+{
+    LevelWork &W = L.W;
+    CgState *st = W.st.as<CgState>();
+    const CgState &hs = L.hs;
+    hipLaunchKernelGGL(k_of_finish, dim3(W.nb), dim3(256), 0, ctx->stream, W.x.as<float4>(),
+                       out, L.n, L.mDx, accumulate ? 1 : 0, st);
+    check_launch("k_of_finish");
+    SolveOut so;
+    so.iters = hs.iters;
+    so.status = hs.status;
+    so.warm = hs.warm;
+    so.resRel = hs.resRel;
+    // ||r||_2 / ||b||_2 of the final (recursive) residual
+    hipLaunchKernelGGL(k_sumsq, dim3(W.nb), dim3(256), 0, ctx->stream,
+                       (const float4 *)W.r.as<float4>(), L.n, W.pdd3());
+    hipLaunchKernelGGL(k_sumsq_fin, dim3(1), dim3(256), 0, ctx->stream, (const DD *)W.pdd3(),
+                       W.nb, &st->rr);
+    check_launch("k_sumsq");
+    CgState *slots = (CgState *)ctx->host_scratch;
+    FLOF_HIP(hipMemcpyAsync(&slots[0], st, sizeof(CgState), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    so.resL2 = slots[0].bb > 0. ? std::sqrt(slots[0].rr / slots[0].bb) : 0.;
+    return so;
+}
+
+// Solves one level.  vin: incoming flow for the rhs (may be null = zero).
+// The result x/h is written to (accumulate ? += : =) out.
+SolveOut solve_level(flof_ctx *ctx, const void *i0, const void *i1, const float4 *vin,
+                     float *rhsT, const Dims4 &D, float wSmooth, float wEnergy,
+                     float cgAccuracy, float dt, int maxIter, const float4 *warm,
+                     const float4 *warm_base, float4 *out, bool accumulate)
+{
+    LevelCg L(ctx, D);
+    level_setup(ctx, L, i0, i1, vin, rhsT, wSmooth, wEnergy, cgAccuracy, dt, maxIter, warm,
+                warm_base, false);
+    LevelWork &W = L.W;
+    CgState *st = W.st.as<CgState>();
+    const int64_t n = L.n;
+    const float dxf = L.dxf;
+    const int pstore = L.pstore;
+    void **pb = L.pb;
+    const LevelGeom G = L.G;  // captured by value in the graph
+    const int nb = W.nb;
+    CgState hs = L.hs;
     // Iterations are enqueued in chunks of CHUNK (as one captured hipGraph
     // unless per-kernel timing is on); after each chunk the state is copied to
     // pinned host memory behind an event, and the host waits only for the
@@ -2117,12 +2273,6 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
                     (const CgState *)st);
         }
     };
-    // the state after the set-up (and warm start), which also says whether
-    // the fast reciprocal is exact on this level; G is captured by value
-    CgState hs;
-    FLOF_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
-    FLOF_HIP(hipStreamSynchronize(ctx->stream));
-    G.fast_rcp = hs.fast_rcp;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     if (!timed && ctx->use_graphs) {
@@ -2172,25 +2322,205 @@ SolveOut solve_level(flof_ctx *ctx, const float *i0, const float *i1, const floa
     if (graph) (void)hipGraphDestroy(graph);
     FLOF_HIP(loop_err);
     check_launch("k_cg_*");
-    hipLaunchKernelGGL(k_of_finish, dim3(W.nb), dim3(256), 0, ctx->stream, W.x.as<float4>(),
-                       out, n, mDx, accumulate ? 1 : 0, st);
-    check_launch("k_of_finish");
-    SolveOut so;
-    so.iters = hs.iters;
-    so.status = hs.status;
-    so.warm = hs.warm;
-    so.resRel = hs.resRel;
-    // ||r||_2 / ||b||_2 of the final (recursive) residual
-    hipLaunchKernelGGL(k_sumsq, dim3(W.nb), dim3(256), 0, ctx->stream,
-                       (const float4 *)W.r.as<float4>(), n, W.pdd3());
-    hipLaunchKernelGGL(k_sumsq_fin, dim3(1), dim3(256), 0, ctx->stream, (const DD *)W.pdd3(),
-                       W.nb, &st->rr);
-    check_launch("k_sumsq");
-    FLOF_HIP(hipMemcpyAsync(&slots[0], st, sizeof(CgState), hipMemcpyDeviceToHost,
+    L.hs = hs;
+    return level_finish(ctx, L, out, accumulate);
+}
+
+// ---------------------------------------------------------------------------
+// Fused batch (no reference counterpart; the reference solves one problem at
+// a time): the PCG iterations of several problems' levels share their
+// launches, one per kernel kind and iteration.  Workgroup b of a launch
+// belongs to the problem whose range [off, off + nb) holds it and runs that
+// problem's kernel body as workgroup b - off of nb -- the same work items,
+// partial sums and order as a launch of the problem alone, so each result
+// is bit-identical to the problem's own solve.  The ranges start at
+// multiples of 8, so every problem keeps its workgroups' XCD assignment.
+// Problems that have converged return at once in every launch.
+struct BItem {
+    LevelGeom G;
+    TMarch M;
+    float dxf;
+    int off, nb;  // workgroups of the stencil and direction kernels
+    const void *i1;
+    float4 *r, *x;
+    void *pb[2];  // direction of even / odd iterations
+    CgState *st;
+    DD *pdd, *pdd2;
+    float *pmax;
+};
+
+__device__ __forceinline__ int bitem_of(const BItem *it, int P, int b)
+{
+    int j = 0;
+    while (j + 1 < P && b >= it[j + 1].off) ++j;
+    return j;
+}
+
+template <class PS, class IM>
+__global__ void __launch_bounds__(256) kb_cg_pq(const BItem *__restrict__ it, int P, int par)
+{
+    const BItem &I = it[bitem_of(it, P, (int)blockIdx.x)];
+    cg_pq_tm_body<PS, IM>((int)blockIdx.x - I.off, I.nb, (const typename PS::T *)I.pb[par],
+                          (const typename IM::T *)I.i1, I.G, I.M, I.dxf, I.st, I.pdd2);
+}
+
+template <class PS, class IM>
+__global__ void __launch_bounds__(256) kb_cg_update(const BItem *__restrict__ it, int P, int par)
+{
+    const BItem &I = it[bitem_of(it, P, (int)blockIdx.x)];
+    cg_update_tm_body<PS, IM>((int)blockIdx.x - I.off, I.nb, I.r,
+                              (const typename PS::T *)I.pb[par], (const typename IM::T *)I.i1,
+                              I.G, I.M, I.dxf, I.st, (const DD *)I.pdd2, I.pdd, I.pmax);
+}
+
+__global__ void kb_cg_beta(const BItem *__restrict__ it)
+{
+    const BItem &I = it[blockIdx.x];
+    cg_beta_body(I.st, I.pdd, I.pmax, I.nb);
+}
+
+template <class PS, bool ODD, class IM>
+__global__ void __launch_bounds__(256) kb_cg_dir(const BItem *__restrict__ it, int P, int par)
+{
+    const BItem &I = it[bitem_of(it, P, (int)blockIdx.x)];
+    cg_dir_tm_body<PS, ODD, IM>((int)blockIdx.x - I.off, I.nb, (const typename PS::T *)I.pb[par],
+                                (typename PS::T *)I.pb[par ^ 1], I.x, I.r,
+                                (const typename IM::T *)I.i1, I.G, I.M, I.dxf, I.st);
+}
+
+__global__ void kb_gather_state(const BItem *__restrict__ it, int P, CgState *out)
+{
+    for (int j = threadIdx.x; j < P; j += blockDim.x) out[j] = *it[j].st;
+}
+
+template <class PS, class IM>
+static void enqueue_fused_iter(flof_ctx *ctx, const BItem *tab, int P, int total, int k)
+{
+    const int par = k & 1;
+    hipLaunchKernelGGL((kb_cg_pq<PS, IM>), dim3(total), dim3(256), 0, ctx->stream, tab, P, par);
+    hipLaunchKernelGGL((kb_cg_update<PS, IM>), dim3(total), dim3(256), 0, ctx->stream, tab, P,
+                       par);
+    hipLaunchKernelGGL(kb_cg_beta, dim3(P), dim3(256), 0, ctx->stream, tab);
+    if (par)
+        hipLaunchKernelGGL((kb_cg_dir<PS, true, IM>), dim3(total), dim3(256), 0, ctx->stream,
+                           tab, P, par);
+    else
+        hipLaunchKernelGGL((kb_cg_dir<PS, false, IM>), dim3(total), dim3(256), 0, ctx->stream,
+                           tab, P, par);
+}
+
+// The PCG iterations of every level in Ls (each set up by level_setup with
+// fused = true), until all have converged or reached their cap; the final
+// states go to each level's hs.
+static void fused_iterations(flof_ctx *ctx, std::vector<LevelCg *> &Ls)
+{
+    const int P = (int)Ls.size();
+    if (P == 0) return;
+    std::vector<BItem> h(P);
+    int total = 0, max_iter = 0;
+    bool all_done = true;
+    for (int j = 0; j < P; ++j) {
+        LevelCg &L = *Ls[j];
+        FLOF_REQUIRE(!L.W.lds && L.W.nb_tm % FLOF_NXCD == 0, FLOF_ERR_STATE,
+                     "fused batch: a level is not set up for shared launches");
+        BItem &I = h[j];
+        I.G = L.G;
+        I.M = L.W.tm;
+        I.dxf = L.dxf;
+        I.off = total;
+        I.nb = L.W.nb_tm;
+        I.i1 = L.i1;
+        I.r = L.W.r.as<float4>();
+        I.x = L.W.x.as<float4>();
+        I.pb[0] = L.pb[0];
+        I.pb[1] = L.pb[1];
+        I.st = L.st();
+        I.pdd = L.W.pdd();
+        I.pdd2 = L.W.pdd2();
+        I.pmax = L.W.pmax();
+        total += I.nb;
+        max_iter = std::max(max_iter, L.hs.maxIter);
+        all_done = all_done && L.hs.done;
+    }
+    if (all_done) return;
+    Tmp tab(ctx, (size_t)P * sizeof(BItem));
+    Tmp states(ctx, (size_t)P * sizeof(CgState));
+    FLOF_HIP(hipMemcpyAsync(tab.p, h.data(), (size_t)P * sizeof(BItem), hipMemcpyHostToDevice,
                             ctx->stream));
-    FLOF_HIP(hipStreamSynchronize(ctx->stream));
-    so.resL2 = slots[0].bb > 0. ? std::sqrt(slots[0].rr / slots[0].bb) : 0.;
-    return so;
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));  // h lives on the host stack
+    const BItem *dtab = tab.as<BItem>();
+    const int pstore = Ls[0]->pstore;
+    const bool bf = ctx->img_bf16;
+    const int CHUNK = 16;
+    auto enqueue_chunk = [&]() {
+        for (int k = 0; k < CHUNK; ++k) {
+            if (bf) {
+                if (pstore == FLOF_PSTORE_BF16)
+                    enqueue_fused_iter<PStoreBF16, ImgBF16>(ctx, dtab, P, total, k);
+                else
+                    enqueue_fused_iter<PStoreF32, ImgBF16>(ctx, dtab, P, total, k);
+            } else {
+                if (pstore == FLOF_PSTORE_BF16)
+                    enqueue_fused_iter<PStoreBF16, ImgF32>(ctx, dtab, P, total, k);
+                else
+                    enqueue_fused_iter<PStoreF32, ImgF32>(ctx, dtab, P, total, k);
+            }
+        }
+        check_launch("kb_cg_*");
+    };
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    if (ctx->use_graphs) {
+        FLOF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_chunk();
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        if (ce == hipSuccess && graph) ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ce != hipSuccess) {
+            (void)hipGetLastError();
+            if (graph) (void)hipGraphDestroy(graph);
+            graph = nullptr;
+            exec = nullptr;
+        }
+    }
+    CgState *slots = nullptr;
+    FLOF_HIP(hipHostMalloc((void **)&slots, 2 * (size_t)P * sizeof(CgState), hipHostMallocDefault));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t loop_err = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+    if (loop_err == hipSuccess) loop_err = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    int64_t chunks = 0;
+    const int64_t max_chunks = (int64_t)max_iter / CHUNK + 3;
+    // the host waits for the chunk before the last one, as solve_level does
+    while (loop_err == hipSuccess && !all_done && chunks < max_chunks) {
+        if (exec) {
+            loop_err = hipGraphLaunch(exec, ctx->stream);
+            if (loop_err != hipSuccess) break;
+        } else {
+            enqueue_chunk();
+        }
+        hipLaunchKernelGGL(kb_gather_state, dim3(1), dim3(64), 0, ctx->stream, dtab, P,
+                           states.as<CgState>());
+        loop_err = hipMemcpyAsync(slots + (chunks & 1) * P, states.p, (size_t)P * sizeof(CgState),
+                                  hipMemcpyDeviceToHost, ctx->stream);
+        if (loop_err == hipSuccess) loop_err = hipEventRecord(ev[chunks & 1], ctx->stream);
+        if (loop_err != hipSuccess) break;
+        ++chunks;
+        if (chunks >= 2) {
+            loop_err = hipEventSynchronize(ev[chunks & 1]);
+            if (loop_err != hipSuccess) break;
+            all_done = true;
+            for (int j = 0; j < P; ++j) all_done = all_done && slots[(chunks & 1) * P + j].done;
+        }
+    }
+    if (loop_err == hipSuccess) loop_err = hipStreamSynchronize(ctx->stream);
+    if (loop_err == hipSuccess && chunks >= 1)
+        for (int j = 0; j < P; ++j) Ls[j]->hs = slots[((chunks - 1) & 1) * P + j];
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipHostFree(slots);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    FLOF_HIP(loop_err);
+    check_launch("kb_cg_*");
 }
 
 // ---------------------------------------------------------------------------
@@ -2214,8 +2544,8 @@ static void record(flof_solve_info *info, int level, int step, const SolveOut &s
 }
 
 // level solve with the post-processing of opticalFlowDim (blur, band reset)
-static SolveOut of_level_post(flof_ctx *ctx, float4 *velCurr, const float *i0w,
-                              const float *i1, float *rhsT, const Dims4 &D, float wSmooth,
+static SolveOut of_level_post(flof_ctx *ctx, float4 *velCurr, const void *i0w,
+                              const void *i1, float *rhsT, const Dims4 &D, float wSmooth,
                               float wEnergy, float postVelBlur, float cgAccuracy,
                               float resetBndWidth, float dt, int maxIter,
                               const float4 *warm, const float4 *warm_base)
@@ -2231,12 +2561,52 @@ static SolveOut of_level_post(flof_ctx *ctx, float4 *velCurr, const float *i0w,
     return so;
 }
 
-static float multiscale(flof_ctx *ctx, float4 *vel, const float *i0, const float *i1,
+// Extensions of the multiscale solve (no reference counterpart): a learned
+// warm start of a level's CG from that level's pre-warped i0 and i1, and a
+// copy of a level's pre-warped i0 and increment (training data of the
+// network).  Both act on levels that run one plain solve (multiStep 1, no
+// projection).
+struct LevelHooks {
+    // guess (cells) of the increment of `level`; false: cold start
+    std::function<bool(int level, const void *i0w, const void *i1, const Dims4 &D,
+                       float4 *guess)>
+        predict;
+    int capture_level = -1;
+    float *cap_i0w = nullptr;
+    float4 *cap_inc = nullptr;
+};
+
+static float multiscale(flof_ctx *ctx, float4 *vel, const void *i0, const void *i1,
                         float *rhsT, const Dims4 &D, const MsParams &P, int level, float dt,
-                        flof_solve_info *info, const float4 *warm)
+                        flof_solve_info *info, const float4 *warm,
+                        const LevelHooks *hooks = nullptr)
 {
     const int64_t n = (int64_t)D.st * D.T;
-    const size_t rb = (size_t)n * 4, vb = (size_t)n * 16;
+    // images: FP32, or bfloat16 in the mixed-precision form (projection,
+    // capture and the network's warm start take FP32 images only)
+    const bool bf = ctx->img_bf16;
+    const size_t ib = bf ? 2 : 4;
+    const size_t rb = (size_t)n * ib, vb = (size_t)n * 16;
+    auto img_lsdiff = [&](const void *a, const void *b, const Dims4 &Dl, float corr, int bnd) {
+        return bf ? host_ls_diff_bf16(ctx, a, b, Dl, corr, bnd)
+                  : host_ls_diff(ctx, (const float *)a, (const float *)b, nullptr, Dl, corr,
+                                 bnd);
+    };
+    auto img_pyramid = [&](void *dst, const Dims4 &Dd, const void *src, const Dims4 &Dsrc) {
+        if (bf)
+            launch_pyramid_bf16(ctx, dst, Dd, src, Dsrc);
+        else
+            launch_pyramid(ctx, dst, Dd, src, Dsrc, false, 1.f);
+    };
+    auto img_prewarp = [&](const float4 *v, void *g) {
+        if (bf) {
+            advect_cfl_bf16(ctx, v, g, D, P.cfl, dt);
+            launch_neumann_bf16(ctx, g, D, 0);
+        } else {
+            advect_cfl(ctx, v, g, D, false, P.cfl, 1.f, dt);
+            launch_neumann(ctx, g, D, false, 0);
+        }
+    };
     int multiStep = P.multiStep;
     bool doFinalProject = P.doFinalProject != 0;
     const int resetBnd = P.resetBndWidth > 0 ? int(P.resetBndWidth * (float)D.X) + 1 : 0;
@@ -2252,8 +2622,7 @@ static float multiscale(flof_ctx *ctx, float4 *vel, const float *i0, const float
     }
     Tmp i0w(ctx, rb);
     FLOF_HIP(hipMemcpyAsync(i0w.p, i0, rb, hipMemcpyDeviceToDevice, ctx->stream));
-    if (info && level == 0)
-        info->err_pre = host_ls_diff(ctx, i0, i1, nullptr, D, lsDiffFac, resetBnd);
+    if (info && level == 0) info->err_pre = img_lsdiff(i0, i1, D, lsDiffFac, resetBnd);
 
     if (D.X > P.minGridSize) {
         Dims4 Ds{D.X / 2, D.Y / 2, D.Z / 2, D.T / 2, 0, 0};
@@ -2262,27 +2631,26 @@ static float multiscale(flof_ctx *ctx, float4 *vel, const float *i0, const float
         FLOF_REQUIRE(Ds.X >= 1 && Ds.Y >= 1 && Ds.Z >= 1 && Ds.T >= 1, FLOF_ERR_DIM,
                      "opticalFlowMultiscale4d: pyramid level too small");
         const int64_t ns = (int64_t)Ds.st * Ds.T;
-        Tmp i0s(ctx, (size_t)ns * 4), i1s(ctx, (size_t)ns * 4), vs(ctx, (size_t)ns * 16),
+        Tmp i0s(ctx, (size_t)ns * ib), i1s(ctx, (size_t)ns * ib), vs(ctx, (size_t)ns * 16),
             dbg(ctx, (size_t)ns * 4), ws;
-        launch_pyramid(ctx, i0s.p, Ds, i0, D, false, 1.f);
-        launch_pyramid(ctx, i1s.p, Ds, i1, D, false, 1.f);
+        img_pyramid(i0s.p, Ds, i0, D);
+        img_pyramid(i1s.p, Ds, i1, D);
         launch_pyramid(ctx, vs.p, Ds, vel, D, true, 0.5f);
         if (warm) {
             ws = Tmp(ctx, (size_t)ns * 16);
             launch_pyramid(ctx, ws.p, Ds, warm, D, true, 0.5f);
         }
         launch_fill(ctx, dbg.p, (size_t)ns * 4);
-        multiscale(ctx, vs.as<float4>(), i0s.as<float>(), i1s.as<float>(), dbg.as<float>(),
-                   Ds, P, level + 1, dt, info, warm ? ws.as<float4>() : nullptr);
+        multiscale(ctx, vs.as<float4>(), i0s.p, i1s.p, dbg.as<float>(), Ds, P, level + 1, dt,
+                   info, warm ? ws.as<float4>() : nullptr, hooks);
         launch_pyramid(ctx, vel, D, vs.p, Ds, true, 2.f);
-        if (rhsT) FLOF_HIP(hipMemcpyAsync(rhsT, i0w.p, rb, hipMemcpyDeviceToDevice, ctx->stream));
+        if (rhsT && !bf)
+            FLOF_HIP(hipMemcpyAsync(rhsT, i0w.p, rb, hipMemcpyDeviceToDevice, ctx->stream));
     }
 
-    advect_cfl(ctx, vel, i0w.p, D, false, P.cfl, 1.f, dt);
-    launch_neumann(ctx, i0w.p, D, false, 0);
+    img_prewarp(vel, i0w.p);
     double errCurr = 0.;
-    if (info || multiStep > 1)
-        errCurr = host_ls_diff(ctx, i0w.as<float>(), i1, nullptr, D, lsDiffFac, resetBnd);
+    if (info || multiStep > 1) errCurr = img_lsdiff(i0w.p, i1, D, lsDiffFac, resetBnd);
     if (info && level < FLOF_MAX_LEVELS) info->err_level[level] = errCurr;
 
     bool doProject = false;
@@ -2306,7 +2674,7 @@ static float multiscale(flof_ctx *ctx, float4 *vel, const float *i0, const float
         int steps = multiStep;
         for (int of = 0; of < steps; ++of) {
             vsv[of] = Tmp(ctx, vb);
-            SolveOut so = of_level_post(ctx, vsv[of].as<float4>(), i0w.as<float>(), i1, rhsT,
+            SolveOut so = of_level_post(ctx, vsv[of].as<float4>(), i0w.p, i1, rhsT,
                                         D, P.wSmooth, P.wEnergy, velBlur, P.cgAccuracy,
                                         P.resetBndWidth, dt, P.cgMaxIter, nullptr, nullptr);
             record(info, level, of, so);
@@ -2324,10 +2692,8 @@ static float multiscale(flof_ctx *ctx, float4 *vel, const float *i0, const float
                              make_float4(1.f, 1.f, 1.f, 1.f), n);
             Tmp w2(ctx, rb);
             FLOF_HIP(hipMemcpyAsync(w2.p, i0, rb, hipMemcpyDeviceToDevice, ctx->stream));
-            advect_cfl(ctx, tmpVel.as<float4>(), w2.p, D, false, P.cfl, 1.f, dt);
-            launch_neumann(ctx, w2.p, D, false, 0);
-            const double errC =
-                host_ls_diff(ctx, w2.as<float>(), i1, nullptr, D, lsDiffFac, resetBnd);
+            img_prewarp(tmpVel.as<float4>(), w2.p);
+            const double errC = img_lsdiff(w2.p, i1, D, lsDiffFac, resetBnd);
             FLOF_HIP(hipMemcpyAsync(i0w.p, w2.p, rb, hipMemcpyDeviceToDevice, ctx->stream));
             if (of > 0 && ((float)errC / (float)errLast) > 0.95) {
                 launch_fill(ctx, vsv[of].p, vb);
@@ -2342,15 +2708,40 @@ static float multiscale(flof_ctx *ctx, float4 *vel, const float *i0, const float
             launch_axpy4(ctx, vel, vsv[of].as<float4>(), make_float4(1.f, 1.f, 1.f, 1.f), n);
     } else if (!doProject) {
         const bool post = P.postVelBlur > 0.f || P.resetBndWidth > 0.f;
-        if (!post) {
+        const bool capture = hooks && hooks->capture_level == level;
+        Tmp guess;
+        bool predicted = false;
+        if (hooks && hooks->predict) {
+            guess = Tmp(ctx, vb);
+            predicted = hooks->predict(level, i0w.p, i1, D, guess.as<float4>());
+        }
+        if (capture)
+            FLOF_HIP(hipMemcpyAsync(hooks->cap_i0w, i0w.p, rb, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+        if (predicted || capture) {
+            // the increment on its own (the network's guess is one of the
+            // increment, not of the total flow)
+            float4 *inc = capture ? hooks->cap_inc : nullptr;
+            Tmp own;
+            if (!inc) {
+                own = Tmp(ctx, vb);
+                inc = own.as<float4>();
+            }
+            SolveOut so = of_level_post(
+                ctx, inc, i0w.p, i1, rhsT, D, P.wSmooth, P.wEnergy, P.postVelBlur,
+                P.cgAccuracy, P.resetBndWidth, dt, P.cgMaxIter,
+                predicted ? guess.as<float4>() : warm, predicted ? nullptr : vel);
+            record(info, level, 0, so);
+            launch_axpy4(ctx, vel, inc, make_float4(1.f, 1.f, 1.f, 1.f), n);
+        } else if (!post) {
             // vel += x/h fused into the finish kernel
-            SolveOut so = solve_level(ctx, i0w.as<float>(), i1, nullptr, rhsT, D, P.wSmooth,
+            SolveOut so = solve_level(ctx, i0w.p, i1, nullptr, rhsT, D, P.wSmooth,
                                       P.wEnergy, P.cgAccuracy, dt, P.cgMaxIter, warm, vel,
                                       vel, true);
             record(info, level, 0, so);
         } else {
             Tmp velCurr(ctx, vb);
-            SolveOut so = of_level_post(ctx, velCurr.as<float4>(), i0w.as<float>(), i1, rhsT,
+            SolveOut so = of_level_post(ctx, velCurr.as<float4>(), i0w.p, i1, rhsT,
                                         D, P.wSmooth, P.wEnergy, P.postVelBlur, P.cgAccuracy,
                                         P.resetBndWidth, dt, P.cgMaxIter, warm, vel);
             record(info, level, 0, so);
@@ -2359,7 +2750,9 @@ static float multiscale(flof_ctx *ctx, float4 *vel, const float *i0, const float
     } else {
         Tmp velCurr(ctx, vb), velTmp2(ctx, vb);
         launch_fill(ctx, velTmp2.p, vb);
-        corr_vels_of(ctx, velCurr.as<float4>(), velTmp2.as<float4>(), i0w.as<float>(), i1, D,
+        FLOF_REQUIRE(!bf, FLOF_ERR_ARG, "mixed precision: no projection (FP32 images only)");
+        corr_vels_of(ctx, velCurr.as<float4>(), velTmp2.as<float4>(), i0w.as<float>(),
+                     (const float *)i1, D,
                      projMaxDist, P.postVelBlur, P.resetBndWidth, projMaxIter, dt);
         launch_axpy4(ctx, vel, velTmp2.as<float4>(), make_float4(1.f, 1.f, 1.f, 1.f), n);
         launch_scale4(ctx, velCurr.as<float4>(), make_float4(-1.f, -1.f, -1.f, -1.f), n);
@@ -2368,32 +2761,193 @@ static float multiscale(flof_ctx *ctx, float4 *vel, const float *i0, const float
 
     if (level == 0 && doFinalProject) {
         Tmp velCurr(ctx, vb);
-        corr_vels_of(ctx, velCurr.as<float4>(), vel, i0, i1, D, projMaxDist, 4.f,
+        FLOF_REQUIRE(!bf, FLOF_ERR_ARG, "mixed precision: no projection (FP32 images only)");
+        corr_vels_of(ctx, velCurr.as<float4>(), vel, (const float *)i0, (const float *)i1, D,
+                     projMaxDist, 4.f,
                      P.resetBndWidth, projMaxIter, dt);
     }
     if (level == 0) {
         FLOF_HIP(hipMemcpyAsync(i0w.p, i0, rb, hipMemcpyDeviceToDevice, ctx->stream));
-        advect_cfl(ctx, vel, i0w.p, D, false, P.cfl, 1.f, dt);
-        launch_neumann(ctx, i0w.p, D, false, 0);
-        errCurr = host_ls_diff(ctx, i0w.as<float>(), i1, nullptr, D, lsDiffFac, resetBnd);
+        img_prewarp(vel, i0w.p);
+        errCurr = img_lsdiff(i0w.p, i1, D, lsDiffFac, resetBnd);
         if (info) info->err_final = errCurr;
     }
     return (float)errCurr;
 }
 
-// the direction storage of one call (flof_of_params.precision)
+// The multiscale solve (optflow4d.cpp:936-1173) of n problems at once, for
+// one plain solve per level (multiStep 1, no blur, band, projection or warm
+// start): the pyramids are built first, then the problems' levels are
+// solved from the coarsest level index down, all problems that have a level
+// of that index together, their PCG iterations in shared launches
+// (fused_iterations).  Per problem the operations and their order are those
+// of multiscale(), so every flow and record is the one a single call gives.
+static void multiscale_fused(flof_ctx *ctx, int np, float4 *const *vel_top,
+                             const void *const *i0_top, const void *const *i1_top,
+                             const Dims4 *D_top, const MsParams &P, float dt,
+                             flof_solve_info *const *info, float *errs)
+{
+    const bool bf = ctx->img_bf16;
+    const size_t ib = bf ? 2 : 4;
+    const int resetBnd = 0;  // the band of resetBndWidth is not part of this form
+    const float lsDiffFac = (float)(0.1 / 20.);
+    struct Prob {
+        std::vector<Dims4> D;
+        std::vector<const void *> i0, i1;
+        std::vector<float4 *> vel;
+        std::vector<Tmp> own;
+    };
+    std::vector<Prob> pr(np);
+    int maxL = 0;
+    auto lsdiff = [&](const void *a, const void *b, const Dims4 &Dl) {
+        return bf ? host_ls_diff_bf16(ctx, a, b, Dl, lsDiffFac, resetBnd)
+                  : host_ls_diff(ctx, (const float *)a, (const float *)b, nullptr, Dl,
+                                 lsDiffFac, resetBnd);
+    };
+    auto prewarp = [&](const float4 *v, void *g, const Dims4 &Dl) {
+        if (bf) {
+            advect_cfl_bf16(ctx, v, g, Dl, P.cfl, dt);
+            launch_neumann_bf16(ctx, g, Dl, 0);
+        } else {
+            advect_cfl(ctx, v, g, Dl, false, P.cfl, 1.f, dt);
+            launch_neumann(ctx, g, Dl, false, 0);
+        }
+    };
+    // pyramids: images and the incoming flow (halved) per level
+    for (int j = 0; j < np; ++j) {
+        Prob &q = pr[j];
+        q.D.push_back(D_top[j]);
+        q.i0.push_back(i0_top[j]);
+        q.i1.push_back(i1_top[j]);
+        q.vel.push_back(vel_top[j]);
+        if (info[j]) info[j]->err_pre = lsdiff(i0_top[j], i1_top[j], D_top[j]);
+        while (q.D.back().X > P.minGridSize) {
+            const Dims4 &Dc = q.D.back();
+            Dims4 Ds{Dc.X / 2, Dc.Y / 2, Dc.Z / 2, Dc.T / 2, 0, 0};
+            Ds.sz = (int64_t)Ds.X * Ds.Y;
+            Ds.st = Ds.sz * Ds.Z;
+            FLOF_REQUIRE(Ds.X >= 1 && Ds.Y >= 1 && Ds.Z >= 1 && Ds.T >= 1, FLOF_ERR_DIM,
+                         "opticalFlowMultiscale4d: pyramid level too small");
+            const int64_t ns = (int64_t)Ds.st * Ds.T;
+            q.own.emplace_back(ctx, (size_t)ns * ib);
+            void *a = q.own.back().p;
+            q.own.emplace_back(ctx, (size_t)ns * ib);
+            void *b = q.own.back().p;
+            q.own.emplace_back(ctx, (size_t)ns * 16);
+            float4 *v = q.own.back().as<float4>();
+            if (bf) {
+                launch_pyramid_bf16(ctx, a, Ds, q.i0.back(), Dc);
+                launch_pyramid_bf16(ctx, b, Ds, q.i1.back(), Dc);
+            } else {
+                launch_pyramid(ctx, a, Ds, q.i0.back(), Dc, false, 1.f);
+                launch_pyramid(ctx, b, Ds, q.i1.back(), Dc, false, 1.f);
+            }
+            launch_pyramid(ctx, v, Ds, q.vel.back(), Dc, true, 0.5f);
+            q.D.push_back(Ds);
+            q.i0.push_back(a);
+            q.i1.push_back(b);
+            q.vel.push_back(v);
+        }
+        const int L = (int)q.D.size();
+        maxL = std::max(maxL, L);
+        if (info[j]) {
+            info[j]->levels = std::min(L, (int)FLOF_MAX_LEVELS);
+            for (int l = 0; l < L && l < FLOF_MAX_LEVELS; ++l) {
+                info[j]->dims[l][0] = q.D[l].X;
+                info[j]->dims[l][1] = q.D[l].Y;
+                info[j]->dims[l][2] = q.D[l].Z;
+                info[j]->dims[l][3] = q.D[l].T;
+            }
+        }
+    }
+    // levels from the coarsest index down; a problem joins at its coarsest
+    for (int l = maxL - 1; l >= 0; --l) {
+        std::vector<int> act;
+        for (int j = 0; j < np; ++j)
+            if ((int)pr[j].D.size() > l) act.push_back(j);
+        std::vector<std::unique_ptr<LevelCg>> Ls;
+        std::vector<Tmp> warped;
+        for (int j : act) {
+            Prob &q = pr[j];
+            const Dims4 &D = q.D[l];
+            const int64_t n = (int64_t)D.st * D.T;
+            if (l + 1 < (int)q.D.size())
+                launch_pyramid(ctx, q.vel[l], D, q.vel[l + 1], q.D[l + 1], true, 2.f);
+            warped.emplace_back(ctx, (size_t)n * ib);
+            void *w = warped.back().p;
+            FLOF_HIP(hipMemcpyAsync(w, q.i0[l], (size_t)n * ib, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+            prewarp(q.vel[l], w, D);
+            if (info[j] && l < FLOF_MAX_LEVELS) info[j]->err_level[l] = lsdiff(w, q.i1[l], D);
+            Ls.emplace_back(new LevelCg(ctx, D));
+            level_setup(ctx, *Ls.back(), w, q.i1[l], nullptr, nullptr, P.wSmooth, P.wEnergy,
+                        P.cgAccuracy, dt, P.cgMaxIter, nullptr, nullptr, true);
+        }
+        std::vector<LevelCg *> raw;
+        for (auto &L : Ls) raw.push_back(L.get());
+        fused_iterations(ctx, raw);
+        for (size_t a = 0; a < act.size(); ++a) {
+            const int j = act[a];
+            SolveOut so = level_finish(ctx, *Ls[a], pr[j].vel[l], true);
+            record(info[j], l, 0, so);
+        }
+    }
+    // the error of the finally warped i0 (level 0)
+    for (int j = 0; j < np; ++j) {
+        Prob &q = pr[j];
+        const Dims4 &D = q.D[0];
+        const int64_t n = (int64_t)D.st * D.T;
+        Tmp w(ctx, (size_t)n * ib);
+        FLOF_HIP(hipMemcpyAsync(w.p, q.i0[0], (size_t)n * ib, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+        prewarp(q.vel[0], w.p, D);
+        const double e = lsdiff(w.p, q.i1[0], D);
+        if (info[j]) info[j]->err_final = e;
+        if (errs) errs[j] = (float)e;
+    }
+}
+
+// the storage precisions of one call (flof_of_params.precision: bit 0 the
+// search direction in bfloat16, bit 1 the images in bfloat16 with binary16
+// gradients)
 struct PStoreScope {
     flof_ctx *c;
     int old;
-    PStoreScope(flof_ctx *c_, int v) : c(c_), old(c_->p_store)
+    bool old_img;
+    PStoreScope(flof_ctx *c_, int v) : c(c_), old(c_->p_store), old_img(c_->img_bf16)
     {
-        FLOF_REQUIRE(v == FLOF_PSTORE_F32 || v == FLOF_PSTORE_BF16, FLOF_ERR_ARG,
-                     "precision: FLOF_PSTORE_F32 or FLOF_PSTORE_BF16");
+        FLOF_REQUIRE(v >= 0 && v <= (FLOF_PSTORE_BF16 | FLOF_PREC_IMG_BF16), FLOF_ERR_ARG,
+                     "precision: a combination of FLOF_PSTORE_BF16 and FLOF_PREC_IMG_BF16");
         FLOF_REQUIRE(v == FLOF_PSTORE_F32 || c->use_tmarch, FLOF_ERR_ARG,
                      "precision: reduced storage needs the t-marching kernels");
-        c->p_store = v;
+        c->p_store = (v & FLOF_PSTORE_BF16) ? FLOF_PSTORE_BF16 : FLOF_PSTORE_F32;
+        c->img_bf16 = (v & FLOF_PREC_IMG_BF16) != 0;
     }
-    ~PStoreScope() { c->p_store = old; }
+    ~PStoreScope()
+    {
+        c->p_store = old;
+        c->img_bf16 = old_img;
+    }
+};
+
+// the images of a call as the solve stores them: the caller's FP32 grids,
+// or bfloat16 copies (rounded to nearest even) in the mixed-precision form
+struct SolveImages {
+    Tmp b0, b1;
+    const void *i0, *i1;
+    SolveImages(flof_ctx *ctx, const flof_grid4d *g0, const flof_grid4d *g1)
+    {
+        i0 = g0->data;
+        i1 = g1->data;
+        if (!ctx->img_bf16) return;
+        const int64_t n = g0->n();
+        b0 = Tmp(ctx, (size_t)n * 2);
+        b1 = Tmp(ctx, (size_t)n * 2);
+        launch_to_bf16(ctx, b0.p, (const float *)g0->data, n);
+        launch_to_bf16(ctx, b1.p, (const float *)g1->data, n);
+        i0 = b0.p;
+        i1 = b1.p;
+    }
 };
 
 static MsParams ms_params(const flof_of_params *p)
@@ -2503,7 +3057,8 @@ int flof_optical_flow4d(flof_grid4d *vel, flof_grid4d *i0, flof_grid4d *i1,
     Tmp vin(ctx, (size_t)n * 16);
     FLOF_HIP(hipMemcpyAsync(vin.p, vel->data, (size_t)n * 16, hipMemcpyDeviceToDevice,
                             ctx->stream));
-    SolveOut so = solve_level(ctx, (const float *)i0->data, (const float *)i1->data,
+    SolveImages im(ctx, i0, i1);
+    SolveOut so = solve_level(ctx, im.i0, im.i1,
                               vin.as<float4>(), rhsT ? (float *)rhsT->data : nullptr, D,
                               (float)p->wSmooth, (float)p->wEnergy, (float)p->cgAccuracy, 1.f,
                               p->cgMaxIter, nullptr, nullptr, (float4 *)vel->data, false);
@@ -2552,10 +3107,97 @@ int flof_optical_flow_multiscale4d(flof_grid4d *vel, flof_grid4d *i0, flof_grid4
     flof_solve_info local;
     flof_solve_info *inf = info ? info : &local;
     std::memset(inf, 0, sizeof(*inf));
+    FLOF_REQUIRE(!(ctx->img_bf16 && rhsT), FLOF_ERR_ARG,
+                 "opticalFlowMultiscale4d: rhsT with FP32 images only");
+    SolveImages im(ctx, i0, i1);
+    const float e = multiscale(ctx, (float4 *)vel->data, im.i0, im.i1,
+                               rhsT ? (float *)rhsT->data : nullptr, dims_of(vel), P, p->level,
+                               1.f, inf, p->warmStart ? (const float4 *)warm->data : nullptr);
+    if (err) *err = e;
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    API_CATCH(ctx)
+}
+
+int flof_optical_flow_multiscale4d_net(flof_grid4d *vel, flof_grid4d *i0, flof_grid4d *i1,
+                                       const flof_of_params *p, flof_warmnet *net,
+                                       int levels, flof_solve_info *info, float *err)
+{
+    flof_ctx *ctx = vel ? vel->ctx : nullptr;
+    API_TRY
+    need4(vel, FLOF_VEC4, "opticalFlowMultiscale4d(vel)");
+    need4(i0, FLOF_REAL, "opticalFlowMultiscale4d(i0)");
+    need4(i1, FLOF_REAL, "opticalFlowMultiscale4d(i1)");
+    need_dims(vel, i0, "opticalFlowMultiscale4d");
+    need_dims(vel, i1, "opticalFlowMultiscale4d");
+    FLOF_REQUIRE(p && net, FLOF_ERR_ARG, "opticalFlowMultiscale4d (net): null argument");
+    FLOF_REQUIRE(!p->warmStart, FLOF_ERR_ARG,
+                 "opticalFlowMultiscale4d (net): the network replaces the warm grid");
+    const MsParams P = ms_params(p);
+    PStoreScope ps(ctx, p->precision);
+    flof_solve_info local;
+    flof_solve_info *inf = info ? info : &local;
+    std::memset(inf, 0, sizeof(*inf));
+    LevelHooks hk;
+    FLOF_REQUIRE(!ctx->img_bf16, FLOF_ERR_ARG,
+                 "opticalFlowMultiscale4d (net): FP32 images only");
+    hk.predict = [&](int level, const void *a, const void *b, const Dims4 &D,
+                     float4 *guess) -> bool {
+        if (levels >= 0 && level >= levels) return false;
+        warmnet_infer(ctx, net, (const float *)a, (const float *)b, D, guess);
+        return true;
+    };
     const float e = multiscale(ctx, (float4 *)vel->data, (const float *)i0->data,
-                               (const float *)i1->data, rhsT ? (float *)rhsT->data : nullptr,
-                               dims_of(vel), P, p->level, 1.f, inf,
-                               p->warmStart ? (const float4 *)warm->data : nullptr);
+                               (const float *)i1->data, nullptr, dims_of(vel), P, p->level, 1.f,
+                               inf, nullptr, &hk);
+    if (err) *err = e;
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
+    API_CATCH(ctx)
+}
+
+int flof_optical_flow_multiscale4d_capture(flof_grid4d *vel, flof_grid4d *i0,
+                                           flof_grid4d *i1, const flof_of_params *p,
+                                           int level, flof_grid4d *i0w_out,
+                                           flof_grid4d *inc_out, flof_solve_info *info,
+                                           float *err)
+{
+    flof_ctx *ctx = vel ? vel->ctx : nullptr;
+    API_TRY
+    need4(vel, FLOF_VEC4, "opticalFlowMultiscale4d(vel)");
+    need4(i0, FLOF_REAL, "opticalFlowMultiscale4d(i0)");
+    need4(i1, FLOF_REAL, "opticalFlowMultiscale4d(i1)");
+    need_dims(vel, i0, "opticalFlowMultiscale4d");
+    need_dims(vel, i1, "opticalFlowMultiscale4d");
+    need4(i0w_out, FLOF_REAL, "opticalFlowMultiscale4d (capture i0w)");
+    need4(inc_out, FLOF_VEC4, "opticalFlowMultiscale4d (capture increment)");
+    need_dims(i0w_out, inc_out, "opticalFlowMultiscale4d (capture)");
+    FLOF_REQUIRE(p && level >= 0, FLOF_ERR_ARG, "opticalFlowMultiscale4d (capture): bad argument");
+    FLOF_REQUIRE(!p->warmStart, FLOF_ERR_ARG, "opticalFlowMultiscale4d (capture): no warm start");
+    FLOF_REQUIRE(p->multiStep <= 1, FLOF_ERR_ARG,
+                 "opticalFlowMultiscale4d (capture): one solve per level");
+    // the size of the captured level: halved once per level below the top
+    Dims4 D = dims_of(vel);
+    for (int l = 0; l < level; ++l) {
+        FLOF_REQUIRE(D.X > p->minGridSize, FLOF_ERR_RANGE,
+                     "opticalFlowMultiscale4d (capture): the pyramid has no such level");
+        D = Dims4{D.X / 2, D.Y / 2, D.Z / 2, D.T / 2, 0, 0};
+    }
+    FLOF_REQUIRE(i0w_out->X == D.X && i0w_out->Y == D.Y && i0w_out->Z == D.Z &&
+                     i0w_out->T == D.T,
+                 FLOF_ERR_DIM, "opticalFlowMultiscale4d (capture): grids of another size");
+    const MsParams P = ms_params(p);
+    PStoreScope ps(ctx, p->precision);
+    flof_solve_info local;
+    flof_solve_info *inf = info ? info : &local;
+    std::memset(inf, 0, sizeof(*inf));
+    LevelHooks hk;
+    FLOF_REQUIRE(!ctx->img_bf16, FLOF_ERR_ARG,
+                 "opticalFlowMultiscale4d (capture): FP32 images only");
+    hk.capture_level = level;
+    hk.cap_i0w = (float *)i0w_out->data;
+    hk.cap_inc = (float4 *)inc_out->data;
+    const float e = multiscale(ctx, (float4 *)vel->data, (const float *)i0->data,
+                               (const float *)i1->data, nullptr, dims_of(vel), P, p->level, 1.f,
+                               inf, nullptr, &hk);
     if (err) *err = e;
     FLOF_HIP(hipStreamSynchronize(ctx->stream));
     API_CATCH(ctx)
@@ -2609,9 +3251,8 @@ int flof_optical_flow_multiscale4d_batch(flof_ctx *ctx, int n, flof_grid4d *cons
                 flof_solve_info local;
                 flof_solve_info *inf = infos ? &infos[j] : &local;
                 std::memset(inf, 0, sizeof(*inf));
-                const float e = multiscale(w, (float4 *)vel[j]->data,
-                                           (const float *)i0[j]->data,
-                                           (const float *)i1[j]->data, nullptr,
+                SolveImages im(w, i0[j], i1[j]);
+                const float e = multiscale(w, (float4 *)vel[j]->data, im.i0, im.i1, nullptr,
                                            dims_of(vel[j]), P, p->level, 1.f, inf, nullptr);
                 FLOF_HIP(hipStreamSynchronize(w->stream));
                 if (errs) errs[j] = e;
@@ -2634,6 +3275,61 @@ int flof_optical_flow_multiscale4d_batch(flof_ctx *ctx, int n, flof_grid4d *cons
     for (int j = 0; j < n; ++j)
         FLOF_REQUIRE(rc[j] == FLOF_OK, rc[j],
                      "opticalFlowMultiscale4dBatch: solve " + std::to_string(j) + ": " + msg[j]);
+    API_CATCH(ctx)
+}
+
+int flof_optical_flow_multiscale4d_fused(flof_ctx *ctx, int n, flof_grid4d *const *vel,
+                                         flof_grid4d *const *i0, flof_grid4d *const *i1,
+                                         const flof_of_params *p, flof_solve_info *infos,
+                                         float *errs)
+{
+    API_TRY
+    FLOF_REQUIRE(ctx && n >= 0 && p && (n == 0 || (vel && i0 && i1)), FLOF_ERR_ARG,
+                 "opticalFlowMultiscale4dFused: bad argument");
+    FLOF_REQUIRE(!p->warmStart && p->multiStep <= 1 && p->postVelBlur <= 0. &&
+                     p->resetBndWidth <= 0. && !p->doFinalProject,
+                 FLOF_ERR_ARG,
+                 "opticalFlowMultiscale4dFused: one plain solve per level (multiStep 1, no "
+                 "blur, band, final projection or warm start)");
+    for (int j = 0; j < n; ++j) {
+        need4(vel[j], FLOF_VEC4, "opticalFlowMultiscale4dFused(vel)");
+        need4(i0[j], FLOF_REAL, "opticalFlowMultiscale4dFused(i0)");
+        need4(i1[j], FLOF_REAL, "opticalFlowMultiscale4dFused(i1)");
+        need_dims(vel[j], i0[j], "opticalFlowMultiscale4dFused");
+        need_dims(vel[j], i1[j], "opticalFlowMultiscale4dFused");
+        FLOF_REQUIRE(vel[j]->ctx->device == ctx->device && i0[j]->ctx->device == ctx->device &&
+                         i1[j]->ctx->device == ctx->device,
+                     FLOF_ERR_ARG, "opticalFlowMultiscale4dFused: grids on another device");
+        FLOF_REQUIRE(vel[j]->X <= p->projSizeThresh, FLOF_ERR_ARG,
+                     "opticalFlowMultiscale4dFused: no projection-only levels");
+        for (int m = 0; m < j; ++m)
+            FLOF_REQUIRE(vel[m] != vel[j], FLOF_ERR_ARG,
+                         "opticalFlowMultiscale4dFused: one vel grid in two solves");
+    }
+    const MsParams P = ms_params(p);
+    if (n == 0) return FLOF_OK;
+    for (int j = 0; j < n; ++j)
+        for (flof_grid4d *g : {vel[j], i0[j], i1[j]})
+            if (g->ctx != ctx) FLOF_HIP(hipStreamSynchronize(g->ctx->stream));
+    PStoreScope ps(ctx, p->precision);
+    std::vector<SolveImages> ims;
+    ims.reserve(n);
+    std::vector<float4 *> vp(n);
+    std::vector<const void *> a(n), b(n);
+    std::vector<Dims4> D(n);
+    std::vector<flof_solve_info> local(n);
+    std::vector<flof_solve_info *> inf(n);
+    for (int j = 0; j < n; ++j) {
+        ims.emplace_back(ctx, i0[j], i1[j]);
+        vp[j] = (float4 *)vel[j]->data;
+        a[j] = ims.back().i0;
+        b[j] = ims.back().i1;
+        D[j] = dims_of(vel[j]);
+        inf[j] = infos ? &infos[j] : &local[j];
+        std::memset(inf[j], 0, sizeof(flof_solve_info));
+    }
+    multiscale_fused(ctx, n, vp.data(), a.data(), b.data(), D.data(), P, 1.f, inf.data(), errs);
+    FLOF_HIP(hipStreamSynchronize(ctx->stream));
     API_CATCH(ctx)
 }
 

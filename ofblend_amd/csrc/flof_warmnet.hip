@@ -556,6 +556,37 @@ void forward(flof_warmnet *net, const float *i0, const float *i1, const Geo &g0,
 
 }  // namespace
 
+void flof::warmnet_infer(flof_ctx *ctx, flof_warmnet *net, const float *i0, const float *i1,
+                         const Dims4 &D, float4 *flow)
+{
+    FLOF_REQUIRE(net && net->ctx == ctx, FLOF_ERR_ARG,
+                 "warm-start net: the network belongs to another context");
+    const int64_t n = D.st * D.T;
+    FLOF_REQUIRE(n >= 1 && n < (1ll << 30), FLOF_ERR_DIM,
+                 "warm-start net: 1 to 2^30 cells");
+    const Geo g0{D.X, D.Y, D.Z, D.T, (int)n, 1};
+    if (!net->frag_ok) pack_all(net);
+    const size_t ab = (size_t)n * C * 4;
+    Tmp h(ctx, ab), h2(ctx, ab), s(ctx, ab), a(ctx, ab);
+    hipLaunchKernelGGL(k_lift, dim3(grid_blocks(ctx, n)), dim3(256), 0, ctx->stream, i0, i1,
+                       (const float *)net->P, h.as<float>(), (int)n);
+    check_launch("k_lift");
+    for (int b = 0; b < NB; ++b) {
+        const float *blk = net->P + OFF_BLK + b * BLK_SIZE;
+        const bf16x8 *f = net->frag + (size_t)b * FRAG_BLK;
+        Geo g = g0;
+        g.dil = 1 << b;
+        conv<true>(ctx, h.as<float>(), s.as<float>(), a.as<float>(), f, blk + BLK_BS, nullptr,
+                   nullptr, g, 1, EP_BIAS);
+        conv<false>(ctx, a.as<float>(), h2.as<float>(), nullptr, f + 2 * FRAG_S, blk + BLK_BT,
+                    nullptr, h.as<float>(), g, 1, EP_BIAS | EP_RES);
+        std::swap(h, h2);
+    }
+    hipLaunchKernelGGL(k_head, dim3(grid_blocks(ctx, n)), dim3(256), 0, ctx->stream,
+                       (const float *)h.as<float>(), (const float *)net->P, flow, (int)n);
+    check_launch("k_head");
+}
+
 
 namespace {
 

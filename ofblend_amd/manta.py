@@ -268,9 +268,11 @@ def opticalFlowMultiscale4d(vel, i0, i1, rhsT=None, wSmooth=0., wEnergy=0.,
                             minGridSize=10, doFinalProject=False, warm=None,
                             cgMaxIter=1000, precision=0):
     """optflow4d.cpp:2182-2195.  Extensions: `warm` is an initial guess of
-    the total flow for the CG of every level; `precision` the storage of the
-    CG search direction (FLOF_PSTORE_F32 = reference arithmetic,
-    FLOF_PSTORE_BF16)."""
+    the total flow for the CG of every level; `precision` the storage
+    precisions, a combination of FLOF_PSTORE_BF16 (the CG search direction in
+    bfloat16) and FLOF_PREC_IMG_BF16 (the images in bfloat16 and the
+    gradients of i1 rounded to binary16); 0 is the reference's FP32
+    arithmetic."""
     if i0.parent.getDt() != 1.0:
         raise FlofError(-6, "Invalid, only dt 1 for now!")
     p = _params(wSmooth=wSmooth, wEnergy=wEnergy, level=level,
@@ -305,6 +307,24 @@ def opticalFlowMultiscale4dBatch(vels, i0s, i1s, concurrency=4, **kw):
     errs = (C.c_float * max(1, n))()
     check(_lib.lib().flof_optical_flow_multiscale4d_batch(
         ctx, n, arr(vels), arr(i0s), arr(i1s), C.byref(p), concurrency, infos, errs), ctx)
+    return [errs[j] for j in range(n)], [infos[j] for j in range(n)]
+
+
+def opticalFlowMultiscale4dFused(vels, i0s, i1s, **kw):
+    """Extension: independent opticalFlowMultiscale4d solves (sizes may
+    differ; one plain solve per level) whose PCG iterations share their
+    launches level by level; returns (errs, infos), each result identical
+    to a single call with the same keyword arguments."""
+    n = len(vels)
+    if not (len(i0s) == n and len(i1s) == n):
+        raise FlofError(-1, "opticalFlowMultiscale4dFused: list lengths differ")
+    p = _params(**{k: (int(v) if k == "doFinalProject" else v) for k, v in kw.items()})
+    ctx = vels[0].ctx if n else context()
+    arr = lambda gs: (C.c_void_p * max(1, n))(*[g.h.value for g in gs])
+    infos = (SolveInfo * max(1, n))()
+    errs = (C.c_float * max(1, n))()
+    check(_lib.lib().flof_optical_flow_multiscale4d_fused(
+        ctx, n, arr(vels), arr(i0s), arr(i1s), C.byref(p), infos, errs), ctx)
     return [errs[j] for j in range(n)], [infos[j] for j in range(n)]
 
 
@@ -751,6 +771,89 @@ def weights2dim(uvw):
         w_uw, w_vw, w_uvw = world2uvw(p, _v3(.5, 0., .5), _v3(0., .5, .5),
                                       _v3(third, third, third))
     return (w0, w1, w2, w_uv, w_vw, w_uw, w_uvw)
+
+
+# --- ofHelpers.py glue (scenes/ofHelpers.py:9-100).  The reference's vec3 /
+# vec4 store single precision, so every assigned component is rounded to
+# float32 here; the arithmetic between assignments is Python (double).
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def vec4ToVec3(vec4d):
+    """ofHelpers.py:9-14."""
+    return vec3(_f32(vec4d[0]), _f32(vec4d[1]), _f32(vec4d[2]))
+
+
+def vec4ToVec3i(vec4d):
+    """ofHelpers.py:16-21: components truncated to int."""
+    return vec3(int(vec4d[0]), int(vec4d[1]), int(vec4d[2]))
+
+
+def vec4intString(vec4d):
+    """ofHelpers.py:23-24."""
+    return "[%d,%d,%d,%d]" % (int(vec4d[0]), int(vec4d[1]), int(vec4d[2]),
+                              int(vec4d[3]))
+
+
+def initRes(res, yFactor, fourdFactor, loadTimeScale):
+    """ofHelpers.py:26-33: (integer 3D grid size, 4D size) of a resolution."""
+    ret = Vec4(_f32(res * 1), _f32(res * yFactor), _f32(res * 1),
+               _f32(res * fourdFactor * loadTimeScale))
+    return (vec4ToVec3i(ret), ret)
+
+
+def getParam(name, default, paramUsed, argv=None):
+    """ofHelpers.py:38-45: the value after `name` on the command line (argv
+    defaults to sys.argv); marks both entries as used."""
+    import sys
+    argv = sys.argv if argv is None else argv
+    while len(paramUsed) < len(argv):
+        paramUsed.append(0)
+    for it in range(1, len(argv)):
+        if argv[it].lower() == name and it + 1 < len(paramUsed):
+            paramUsed[it] = paramUsed[it + 1] = 1
+            return argv[it + 1]
+    return default
+
+
+def checkUnusedParam(paramUsed, argv=None):
+    """ofHelpers.py:47-54: exits with status 1 if a parameter was not used."""
+    import sys
+    argv = sys.argv if argv is None else argv
+    err = False
+    for it in range(1, len(argv)):
+        if paramUsed[it] == 0:
+            print("Error: param %d '%s' not used!" % (it, argv[it]))
+            err = True
+    if err:
+        sys.exit(1)
+
+
+def initOffsets(gs4d, gs4dDefo, autoBorder, repeatStartDist, loadTimeScale):
+    """ofHelpers.py:59-86: (loadOffset, loadScale, defoOffset, defoScale,
+    defoFactor) of the phi load (a border of autoBorder on every side, the
+    start shifted by repeatStartDist in t) and of the deformation (rescaled
+    by the ratio of the two 4D sizes)."""
+    lt = _f32(gs4d[3] * autoBorder)
+    loadOffset = Vec4(_f32(gs4d[0] * autoBorder), _f32(gs4d[1] * autoBorder),
+                      _f32(gs4d[2] * autoBorder), _f32(lt + repeatStartDist))
+    ls = _f32(1. - 2.0 * autoBorder)
+    loadScale = Vec4(ls, ls, ls, _f32(ls * loadTimeScale))
+    defoOffset = Vec4(0., 0., 0., 0.)
+    defoScale = Vec4(1., 1., 1., _f32(loadTimeScale))
+    defoFactor = Vec4(*[_f32(_f32(gs4d[c]) / _f32(gs4dDefo[c])) for c in range(4)])
+    return (loadOffset, loadScale, defoOffset, defoScale, defoFactor)
+
+
+def loadVelRescaled(velgrid, prefix, defoSolver, defoOffset, defoScale, defoFactor):
+    """ofHelpers.py:91-100: "<prefix>_vel.uni" (of the deformation solver's
+    size) resampled into velgrid and scaled by defoFactor, on the GPU."""
+    size = (C.c_int * 4)(*(tuple(defoSolver.gridSize) + (defoSolver.fourthDim,)))
+    check(_lib.lib().flof_load_vel_rescaled(
+        velgrid.h, ("%s_vel.uni" % prefix).encode(), size, floats(*defoOffset),
+        floats(*defoScale), floats(*defoFactor)), velgrid.ctx)
 
 
 def sync():

@@ -83,11 +83,47 @@ class WarmNet:
         check(_lib.lib().flof_warmnet_warm_start(self.h, i0.h, i1.h, int(netX), warm.h),
               self.ctx)
 
+    def solve(self, vel, i0, i1, levels=1, **kw):
+        """opticalFlowMultiscale4d with this network's warm start on the
+        `levels` finest levels (-1: all); returns (err, SolveInfo)."""
+        p = _params_of(kw)
+        info = M.SolveInfo()
+        err = C.c_float()
+        check(_lib.lib().flof_optical_flow_multiscale4d_net(
+            vel.h, i0.h, i1.h, C.byref(p), self.h, int(levels), C.byref(info),
+            C.byref(err)), self.ctx)
+        return err.value, info
+
     def save(self, path):
         np.save(path, self.get_params(), allow_pickle=False)
 
     def load(self, path):
         self.set_params(np.load(path, allow_pickle=False))
+
+
+def _params_of(kw):
+    kw = dict(kw)
+    if "doFinalProject" in kw:
+        kw["doFinalProject"] = int(kw["doFinalProject"])
+    return M.default_params(**kw)
+
+
+def capture(vel, i0, i1, level=0, **kw):
+    """opticalFlowMultiscale4d that also returns the pre-warped i0 of pyramid
+    level `level` and the increment its solve adds (new grids of that
+    level's size): (err, info, i0w, inc)."""
+    X, Y, Z, T = vel.size
+    for _ in range(level):
+        X, Y, Z, T = X // 2, Y // 2, Z // 2, T // 2
+    i0w = M.Grid4Real(vel.parent, (X, Y, Z, T))
+    inc = M.Grid4Vec4(vel.parent, (X, Y, Z, T))
+    p = _params_of(kw)
+    info = M.SolveInfo()
+    err = C.c_float()
+    check(_lib.lib().flof_optical_flow_multiscale4d_capture(
+        vel.h, i0.h, i1.h, C.byref(p), int(level), i0w.h, inc.h, C.byref(info),
+        C.byref(err)), vel.ctx)
+    return err.value, info, i0w, inc
 
 
 PLAIN = dict(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5, multiStep=1, postVelBlur=0.0,

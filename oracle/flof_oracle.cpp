@@ -30,6 +30,7 @@
 #include <cstring>
 #include <limits>
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 
 #include "../include/flof.h"
@@ -159,6 +160,59 @@ template <class R> static inline R round_p(R v)
     return v;
 }
 
+// Mixed-precision images (extension; no reference counterpart): with
+// g_imgmode set, the FP32 solve stores its images (i0, i1, their pyramid
+// levels and every warp of i0) as bfloat16 -- each stored value rounded to
+// nearest even -- and rounds each component of the gradient of i1 to the
+// nearest binary16 value where it is formed.  The FP64 variant is never
+// rounded (it is the accuracy yardstick).
+static int g_imgmode = 0;
+
+// float -> binary16 -> float, round to nearest even, subnormals kept,
+// magnitudes from 65520 on to infinity (the IEEE 754 conversion)
+static float round_f16(float x)
+{
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    const uint32_t sign = u & 0x80000000u;
+    uint32_t a = u & 0x7fffffffu;
+    if (a >= 0x7f800000u) return x;
+    uint32_t r;
+    if (a >= 0x477ff000u) {
+        r = sign | 0x7f800000u;
+    } else if (a < 0x38800000u) {
+        // below 2^-14: multiples of 2^-24 (exact scalings by powers of 2)
+        const float q = std::nearbyint(std::fabs(x) * 16777216.0f) * 5.9604644775390625e-8f;
+        std::memcpy(&r, &q, 4);
+        r |= sign;
+    } else {
+        a += 0x0fffu + ((a >> 13) & 1u);
+        r = sign | (a & ~0x1fffu);
+    }
+    float o;
+    std::memcpy(&o, &r, 4);
+    return o;
+}
+
+template <class R> static inline R img_round(R v)
+{
+    if constexpr (std::is_same<R, float>::value)
+        if (g_imgmode) return round_bf16(v);
+    return v;
+}
+template <class R> static inline R grad_round(R v)
+{
+    if constexpr (std::is_same<R, float>::value)
+        if (g_imgmode) return round_f16(v);
+    return v;
+}
+template <class R> static void img_round_all(R *a, idx_t n)
+{
+    if constexpr (std::is_same<R, float>::value)
+        if (g_imgmode)
+            for (idx_t c = 0; c < n; ++c) a[c] = round_bf16(a[c]);
+}
+
 // ---------------------------------------------------------------------------
 // Quadrilinear sampling, util/vector4d.h:422-516.  Cell centres are at c+1/2;
 // per dimension q = p - 1/2, i = (int)q, w1 = q - i; q < 0 clamps to (0; 1,0),
@@ -264,14 +318,18 @@ template <class R> R maxNormVec(const V4<R> *v, idx_t n)
 // substeps of dt/steps.
 template <class R, class T>
 void advectCfl4d(const V4<R> *vel, T *grid, const Dims &D, R cfl,
-                 R velFactor, R dt)
+                 R velFactor, R dt, bool image = false)
 {
     std::vector<V4<R>> vt(vel, vel + D.n());
     for (auto &v : vt) v = v * V4<R>(velFactor);
     const R maxVel = maxNormVec(vt.data(), D.n()) * dt;
     const int steps = int(maxVel / cfl) + 1;
     const R sdt = dt / R(steps);
-    for (int s = 0; s < steps; ++s) advect4d<R, T>(vt.data(), grid, D, sdt);
+    for (int s = 0; s < steps; ++s) {
+        advect4d<R, T>(vt.data(), grid, D, sdt);
+        if constexpr (std::is_same<T, R>::value)
+            if (image) img_round_all<R>(grid, D.n());
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -643,10 +701,10 @@ CgResult opticalFlowLevel(V4<R> *vel, const R *i0, const R *i1, R *rhsT,
                     }
                     const R tderiv = (i1[c] - i0[c]) / mDt;
                     R g[4];
-                    g[0] = (i1[c + 1] - i1[c - 1]) * dxf;
-                    g[1] = (i1[c + D.X] - i1[c - D.X]) * dxf;
-                    g[2] = (i1[c + D.sz()] - i1[c - D.sz()]) * dxf;
-                    g[3] = (i1[c + D.st()] - i1[c - D.st()]) * dxf;
+                    g[0] = grad_round<R>((i1[c + 1] - i1[c - 1]) * dxf);
+                    g[1] = grad_round<R>((i1[c + D.X] - i1[c - D.X]) * dxf);
+                    g[2] = grad_round<R>((i1[c + D.sz()] - i1[c - D.sz()]) * dxf);
+                    g[3] = grad_round<R>((i1[c + D.st()] - i1[c - D.st()]) * dxf);
                     for (int d = 0; d < 4; ++d) {
                         R r = -g[d] * tderiv;
                         for (int m = 0; m < 8; ++m) {
@@ -871,6 +929,8 @@ R multiscale(V4<R> *vel, const R *i0, const R *i1, R *rhsT, const Dims &D,
         std::vector<V4<R>> vs(Ds.n()), ws;
         pyramidResample<R, R>(i0s.data(), Ds, i0, D);
         pyramidResample<R, R>(i1s.data(), Ds, i1, D);
+        img_round_all<R>(i0s.data(), Ds.n());
+        img_round_all<R>(i1s.data(), Ds.n());
         pyramidResample<R, V4<R>>(vs.data(), Ds, vel, D);
         for (auto &v : vs) v = v * V4<R>((R)0.5);
         if (warm) {
@@ -887,7 +947,7 @@ R multiscale(V4<R> *vel, const R *i0, const R *i1, R *rhsT, const Dims &D,
         if (rhsT) std::copy(i0warped.begin(), i0warped.end(), rhsT);
     }
 
-    advectCfl4d<R, R>(vel, i0warped.data(), D, cfl, (R)1., dt);
+    advectCfl4d<R, R>(vel, i0warped.data(), D, cfl, (R)1., dt, true);
     setBoundNeumann<R>(i0warped.data(), D, 0);
     R errCurr = lsDiff<R>(i0warped.data(), i1, nullptr, D, lsDiffFac, resetBnd);
     if (info && level < FLOF_MAX_LEVELS) info->err_level[level] = errCurr;
@@ -933,7 +993,7 @@ R multiscale(V4<R> *vel, const R *i0, const R *i1, R *rhsT, const Dims &D,
             for (int k = of; k >= 0; --k)
                 for (idx_t c = 0; c < n; ++c) tmpVel[c] = tmpVel[c] + vs2[k][c];
             std::vector<R> w2(i0, i0 + n);
-            advectCfl4d<R, R>(tmpVel.data(), w2.data(), D, cfl, (R)1., dt);
+            advectCfl4d<R, R>(tmpVel.data(), w2.data(), D, cfl, (R)1., dt, true);
             setBoundNeumann<R>(w2.data(), D, 0);
             const R errC = lsDiff<R>(w2.data(), i1, nullptr, D, lsDiffFac, resetBnd);
             i0warped = w2;
@@ -983,7 +1043,7 @@ R multiscale(V4<R> *vel, const R *i0, const R *i1, R *rhsT, const Dims &D,
     }
     if (level == 0) {
         std::copy(i0, i0 + n, i0warped.begin());
-        advectCfl4d<R, R>(vel, i0warped.data(), D, cfl, (R)1., dt);
+        advectCfl4d<R, R>(vel, i0warped.data(), D, cfl, (R)1., dt, true);
         setBoundNeumann<R>(i0warped.data(), D, 0);
         errCurr = lsDiff<R>(i0warped.data(), i1, nullptr, D, lsDiffFac, resetBnd);
         if (info) info->err_final = errCurr;
@@ -1504,9 +1564,15 @@ using namespace orc;
         const flof_of_params *p, double dt, const R *warm, int32_t *iters,     \
         double *residual, int32_t *status)                                     \
     {                                                                          \
-        g_pstore = p->precision;                                               \
+        g_pstore = p->precision & 1;                                           \
+        g_imgmode = (p->precision & 2) != 0;                                   \
+        const idx_t nn = (idx_t)X * Y * Z * T;                                 \
+        std::vector<R> b0(i0, i0 + nn), b1(i1, i1 + nn);                       \
+        img_round_all<R>(b0.data(), nn);                                       \
+        img_round_all<R>(b1.data(), nn);                                       \
         CgResult cg = opticalFlowLevel<R>(                                     \
-            (V4<R> *)vel, i0, i1, rhsT, Dims{X, Y, Z, T}, (R)p->wSmooth,       \
+            (V4<R> *)vel, b0.data(), b1.data(), rhsT, Dims{X, Y, Z, T},        \
+            (R)p->wSmooth,                                                     \
             (R)p->wEnergy, (R)p->postVelBlur, (R)p->cgAccuracy,                \
             (R)p->resetBndWidth, (R)dt, p->cgMaxIter, (const V4<R> *)warm);    \
         if (iters) *iters = cg.iters;                                          \
@@ -1520,8 +1586,13 @@ using namespace orc;
     {                                                                          \
         if (info) std::memset(info, 0, sizeof(*info));                         \
         OfParams P = fromC(p);                                                 \
-        g_pstore = p->precision;                                               \
-        return (double)multiscale<R>((V4<R> *)vel, i0, i1, rhsT,               \
+        g_pstore = p->precision & 1;                                           \
+        g_imgmode = (p->precision & 2) != 0;                                   \
+        const idx_t nn = (idx_t)X * Y * Z * T;                                 \
+        std::vector<R> b0(i0, i0 + nn), b1(i1, i1 + nn);                       \
+        img_round_all<R>(b0.data(), nn);                                       \
+        img_round_all<R>(b1.data(), nn);                                       \
+        return (double)multiscale<R>((V4<R> *)vel, b0.data(), b1.data(), rhsT, \
                                      Dims{X, Y, Z, T}, P, p->level, (R)dt,     \
                                      info, (const V4<R> *)warm);               \
     }                                                                          \
@@ -1620,6 +1691,7 @@ using namespace orc;
 
 extern "C" void orc_set_sum_mode(int mode) { g_sum_mode = mode; }
 extern "C" float orc_round_bf16(float f) { return round_bf16(f); }
+extern "C" float orc_round_f16(float f) { return round_f16(f); }
 extern "C" int orc_get_sum_mode(void) { return g_sum_mode; }
 
 ORC_DEFINE(float, _f32)

@@ -82,7 +82,7 @@ def _declare(L):
     L.orc_set_sum_mode.restype = None
     L.orc_get_sum_mode.argtypes = []
     L.orc_get_sum_mode.restype = C.c_int
-    for name in ("orc_round_bf16",):
+    for name in ("orc_round_bf16", "orc_round_f16"):
         getattr(L, name).argtypes = [C.c_float]
         getattr(L, name).restype = C.c_float
     for suf in ("_f32", "_f64"):
@@ -137,6 +137,12 @@ def round_bf16(x):
     """The oracle's float -> bfloat16 -> float rounding (search-direction
     storage precision 1)."""
     return lib().orc_round_bf16(float(x))
+
+
+def round_f16(x):
+    """The oracle's float -> binary16 -> float rounding (the gradients of the
+    mixed-precision solve)."""
+    return lib().orc_round_f16(float(x))
 
 
 def get_sum_mode():

@@ -5,6 +5,7 @@ import pytest
 
 from oracle import oracle as O
 from ofblend_amd import manta as M
+from ofblend_amd.abi import default_params
 
 pytestmark = pytest.mark.gpu
 
@@ -63,3 +64,78 @@ def test_autotune_keeps_results():
         assert L.flof_ctx_tuned_blocks(ctx, X, Y, Z, T) in (2, 4, 8, 16)
     finally:
         L.flof_ctx_set_autotune(ctx, 0)
+
+
+def _jobs(shapes, seed0, nspheres=1):
+    jobs = []
+    for n, (X, Y, Z, T) in enumerate(shapes):
+        s = M.Solver(gridSize=M.vec3(X, Y, Z), fourthDim=T)
+        a = O.drop_sdf(X, Y, Z, T, seed=seed0 + 2 * n, nspheres=nspheres, rmin=0.08,
+                       rmax=0.18).astype(np.float32)
+        b = O.drop_sdf(X, Y, Z, T, seed=seed0 + 1 + 2 * n, nspheres=nspheres, rmin=0.08,
+                       rmax=0.18).astype(np.float32)
+        g0, g1, v = s.create(M.Grid4Real), s.create(M.Grid4Real), s.create(M.Grid4Vec4)
+        g0.from_numpy(a)
+        g1.from_numpy(b)
+        jobs.append((s, g0, g1, v, a, b))
+    return jobs
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_fused_batch_matches_oracle(precision):
+    """The fused batch (shared PCG launches per level) on uneven small
+    problems with different pyramid depths (3, 2 and 1 levels): each flow,
+    iteration record and error is the oracle's (correctly rounded sums), bit
+    for bit; also in the mixed-precision form."""
+    shapes = [(20, 18, 16, 14), (12, 12, 12, 10), (8, 9, 10, 11), (17, 16, 16, 16)]
+    jobs = _jobs(shapes, 40, nspheres=2)
+    prm = dict(P, minGridSize=6)
+    print("fused batch vs oracle, precision", precision, flush=True)
+    errs, infos = M.opticalFlowMultiscale4dFused([j[3] for j in jobs], [j[1] for j in jobs],
+                                                 [j[2] for j in jobs], precision=precision,
+                                                 **prm)
+    for n, (s, g0, g1, v, a, b) in enumerate(jobs):
+        with O.sum_mode(1):
+            ref, rerr, rinfo = O.multiscale4d(np.zeros(a.shape + (4,), np.float32), a, b,
+                                              default_params(precision=precision, **prm))
+        got = infos[n].summary()["levels"]
+        want = rinfo.summary()["levels"]
+        print(n, shapes[n], "iters", [l["iters"] for l in got], flush=True)
+        assert [l["iters"] for l in got] == [l["iters"] for l in want]
+        assert [l["dims"] for l in got] == [l["dims"] for l in want]
+        assert np.array_equal(v.to_numpy(), ref), n
+        assert errs[n] == np.float32(rerr)
+
+
+def test_fused_batch_equals_single_solves():
+    """Larger uneven problems (forward and backward pairs): the fused batch
+    gives exactly the flows, records and errors of single solves."""
+    shapes = [(48, 48, 48, 32), (40, 40, 40, 24), (32, 32, 32, 48)]
+    jobs = _jobs(shapes, 60)
+    jobs += [(s, g1, g0, s.create(M.Grid4Vec4), b, a) for (s, g0, g1, v, a, b) in jobs]
+    prm = dict(P, minGridSize=16)
+    ref = []
+    for s, g0, g1, v, a, b in jobs:
+        v.clear()
+        e = M.opticalFlowMultiscale4d(vel=v, i0=g0, i1=g1, **prm)
+        ref.append((e, v.to_numpy(), M.getLastSolveInfo().summary()))
+        v.clear()
+    print("fused batch vs single solves", len(jobs), flush=True)
+    errs, infos = M.opticalFlowMultiscale4dFused([j[3] for j in jobs], [j[1] for j in jobs],
+                                                 [j[2] for j in jobs], **prm)
+    for n, j in enumerate(jobs):
+        e, vr, inf = ref[n]
+        assert errs[n] == e
+        assert np.array_equal(j[3].to_numpy(), vr), n
+        assert infos[n].summary()["levels"] == inf["levels"]
+
+
+def test_fused_batch_rejects_other_parameters():
+    s = M.Solver(gridSize=M.vec3(8, 8, 8), fourthDim=8)
+    g0, g1, v = s.create(M.Grid4Real), s.create(M.Grid4Real), s.create(M.Grid4Vec4)
+    for bad in (dict(multiStep=3), dict(postVelBlur=2.0), dict(resetBndWidth=0.1),
+                dict(doFinalProject=True)):
+        prm = dict(P)
+        prm.update(bad)
+        with pytest.raises(M.FlofError):
+            M.opticalFlowMultiscale4dFused([v], [g0], [g1], **prm)

@@ -196,3 +196,47 @@ def test_reductions_mult_and_vec4_region():
     ref = v.copy()
     ref[0:3, 1:4, 2:5, 1:6] = np.float32([1, 2, 3, 4])
     assert np.array_equal(gv.to_numpy(), ref)
+
+
+@pytest.mark.parametrize("vec", [True, False])
+def test_gaussian_blur_two_x_tiles_bit_exact(vec):
+    """gaussianBlur4d at sigma 1 (the LDS column march over 64-wide x tiles)
+    on a grid 70 wide: the seam between the two x tiles and the ragged second
+    tile equal the oracle bit for bit, Vec4 and Real."""
+    X, Y, Z, T = 70, 9, 10, 11
+    rng = np.random.default_rng(12)
+    shp = (T, Z, Y, X, 4) if vec else (T, Z, Y, X)
+    a = rng.standard_normal(shp).astype(np.float32)
+    s = M.Solver(gridSize=M.vec3(X, Y, Z), fourthDim=T)
+    g = s.create(M.Grid4Vec4 if vec else M.Grid4Real)
+    g.from_numpy(a)
+    say("gaussian blur 70x9x10x11 vec", vec)
+    M.gaussianBlur4d(g, 1.0, 1)
+    assert np.array_equal(g.to_numpy(), O.gaussian_blur(a, 1.0, 1))
+
+
+@pytest.mark.parametrize("vx", [0.0, 1.0])
+def test_corr_vels_of4d_two_x_tiles_bit_exact(vx):
+    """corrVelsOf4d on a grid 70 wide (two x tiles of the box passes): 4 and
+    5 extrapolation passes, random level sets, bit-exact to the oracle."""
+    X, Y, Z, T = 70, 10, 9, 8
+    rng = np.random.default_rng(13)
+    t, z, y, x = np.meshgrid(*[np.arange(s) + 0.5 for s in (T, Z, Y, X)], indexing="ij")
+    base = x - 0.45 * X + 0.3 * np.sin(0.7 * y) + 0.2 * np.cos(0.5 * z + 0.3 * t)
+    phiOrg = (base + 0.05 * rng.standard_normal(base.shape)).astype(np.float32)
+    phiTgt = (base - 1.7 + 0.05 * rng.standard_normal(base.shape)).astype(np.float32)
+    vel0 = np.zeros((T, Z, Y, X, 4), np.float32)
+    vel0[..., 0] = vx
+    s = M.Solver(gridSize=M.vec3(X, Y, Z), fourthDim=T)
+    dst, vel = s.create(M.Grid4Vec4), s.create(M.Grid4Vec4)
+    vel.from_numpy(vel0)
+    po, pt = s.create(M.Grid4Real), s.create(M.Grid4Real)
+    po.from_numpy(phiOrg)
+    pt.from_numpy(phiTgt)
+    say("corrVelsOf4d 70 wide vx", vx)
+    M.corrVelsOf4d(dst, vel, po, po, pt, threshPhi=4.0, postVelBlur=0.0, maxIter=40)
+    rd, rv = O.corr_vels_of4d(vel0, phiOrg, phiTgt, threshPhi=4.0, postVelBlur=0.0,
+                              maxIter=40)
+    assert np.abs(rd).max() > 0.5
+    assert np.array_equal(dst.to_numpy(), rd)
+    assert np.array_equal(vel.to_numpy(), rv)

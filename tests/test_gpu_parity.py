@@ -8,10 +8,13 @@ bit-identical; against the serial order of the reference (sum_mode(0)) the
 FP32 CG result differs by the reference's own sensitivity to summation order,
 bounded below.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
 from oracle import oracle as O
+from ofblend_amd import _lib
 from ofblend_amd import manta as M
 from ofblend_amd.abi import default_params
 
@@ -242,6 +245,73 @@ def test_multiscale_golden_digest(case):
     assert iters == d["iters"]
     assert hashlib.sha256(v.tobytes()).hexdigest() == d["sha256"]
     assert err == np.float32(d["err"])
+    if "distances" in d:
+        # the flow is the sum_mode-1 oracle's bit for bit, so the recorded
+        # distances are the GPU's: to the reference's serial order and, for
+        # both, to the FP64 oracle
+        dd = d["distances"]
+        say(case, "max|gpu - serial order|", dd["gpu_vs_serial"]["max_abs_diff"],
+            "max|serial - fp64|", dd["serial_vs_fp64"]["max_abs_diff"],
+            "max|gpu - fp64|", dd["gpu_vs_fp64"]["max_abs_diff"])
+        if d["fp64_oracle"]["iters"] == d["iters"]:  # every level converged alike
+            assert dd["gpu_vs_serial"]["max_abs_diff"] <= dd["serial_vs_fp64"]["max_abs_diff"]
+            assert dd["gpu_vs_fp64"]["rms_diff"] <= 1.1 * dd["serial_vs_fp64"]["rms_diff"]
+    if "warp" in d:
+        _golden_warp(d, vel, N)
+
+
+def _golden_warp(d, vel, N):
+    """The frame warp of the golden case on the GPU: the drops as N frames at
+    twice the resolution, placed 1:1 onto t, all N frames warped by the
+    solved flow in one launch (alpha 1, defoFactor (2, 2, 2, 1), zeroed
+    frames); the first, middle and last frame match the oracle's digest, and
+    the middle one the oracle's warp of the GPU flow directly."""
+    import hashlib
+    w = d["warp"]
+    F = w["scale"]
+    frames = O.drop_sdf(N, N, N, N, seed=1, scale=F)
+    sp = solver(N * F, N * F, N * F, N)
+    phi = sp.create(M.Grid4Real)
+    M.loadPlaceGrid4dFrames(M.DeviceArray.from_numpy(frames), phi, M.Vec4(0.), M.Vec4(1.))
+    del frames
+    dst = M.DeviceArray.from_numpy(np.zeros((N, N * F, N * F, N * F), np.float32))
+    f4 = lambda *c: (C.c_float * 4)(*c)
+    L = _lib.lib()
+    say("warp", N, "frames of", N * F, "^3")
+    _lib.check(L.flof_advect_time_slices_opt(
+        vel.h, dst.ptr, N * F, N * F, N * F, N, 0.0, 1.0, phi.h, 1.0, 1.0,
+        f4(0, 0, 0, 0), f4(1, 1, 1, 1), f4(2, 2, 2, 1), 1, 1.0), vel.ctx)
+    out = dst.to_numpy()
+    sel = np.stack([out[t] for t in w["frames"]])
+    assert hashlib.sha256(sel.tobytes()).hexdigest() == w["sha256"]
+    mid = w["frames"][1]
+    ref = O.warp_slice(vel.to_numpy(), phi.to_numpy(), np.zeros(out.shape[1:], np.float32),
+                       float(mid), blendAlpha=1.0, defoFactor=(2, 2, 2, 1))
+    assert np.array_equal(out[mid], ref)
+    say("warp max|gpu - fp64|", w["gpu_vs_fp64"]["max_abs_diff"],
+        "max|serial - fp64|", w["serial_vs_fp64"]["max_abs_diff"])
+
+
+@pytest.mark.parametrize("N", [16, 24])
+def test_parity_against_reference_order(N):
+    """The reference sums its CG dot products serially in FP64
+    (optflow4d.cpp:234-241); the GPU sums them correctly rounded.  Against
+    the oracle in the reference's order, the GPU flow is closer than that
+    order's own FP32 flow is to the FP64 oracle, and the GPU's distance to
+    the FP64 oracle matches the reference order's (rms within 10%)."""
+    v, err, info, (i0, i1, p) = _multiscale_case(N, N, N // 4)
+    with O.sum_mode(0):
+        ser, _, sinfo = O.multiscale4d(np.zeros(v.shape, np.float32), i0, i1, p)
+        d0 = O.drop_sdf(N, N, N, N, seed=1, dtype=np.float64)
+        d1 = O.drop_sdf(N, N, N, N, seed=2, dtype=np.float64)
+        v64, _, _ = O.multiscale4d(np.zeros(v.shape, np.float64), d0, d1, p)
+    gs = np.abs(v.astype(np.float64) - ser).max()
+    s64 = np.abs(ser - v64).max()
+    g64 = np.abs(v - v64)
+    say(N, "max|gpu - serial|", gs, "max|serial - fp64|", s64, "max|gpu - fp64|", g64.max())
+    assert gs <= s64
+    rms = lambda a: float(np.sqrt((a ** 2).mean()))
+    assert rms(g64) <= 1.1 * rms(ser - v64)
 
 
 def test_frames_place_and_warp_bit_exact():

@@ -223,3 +223,29 @@ def test_advect_time_slice_defo_volumes(tmp_path, nv, aligned):
         assert np.array_equal(dst.to_numpy(), ref)
     finally:
         M.loadAdvectTimeSlice_Finish(3)
+
+
+def test_load_vel_rescaled(tmp_path):
+    """loadVelRescaled (ofHelpers.py:91-100) as flof.py calls it: a
+    deformation solved at a coarser resolution is resampled to the phi grid
+    (interpolateGrid4dVec with the initOffsets defoScale) and scaled by
+    defoFactor -- against the oracle's resample and an FP32 product."""
+    say("loadVelRescaled")
+    gsd = M.initRes(8, 1, 1.5, 1.)[1]
+    gs = M.initRes(12, 1, 1.5, 1.)[1]
+    (_, _, dOff, dScale, dFac) = M.initOffsets(gs, gsd, 0.1, 1.2, 1.)
+    Xd, Yd, Zd, Td = [int(v) for v in gsd]
+    v = (RNG.standard_normal((Td, Zd, Yd, Xd, 4)) * 2).astype(np.float32)
+    M.writeUni4d(str(tmp_path / "pair_vel.uni"), v)
+    sd = solver(Xd, Yd, Zd, Td)
+    s = solver(*[int(c) for c in gs])
+    vel = s.create(M.Grid4Vec4)
+    M.loadVelRescaled(vel, str(tmp_path / "pair"), sd, dOff, dScale, dFac)
+    ref = O.interpolate_grid4d(vel.np_shape()[:4], v, offset=dOff, scale=dScale)
+    ref = (ref * np.asarray(dFac, dtype=np.float32)).astype(np.float32)
+    assert np.array_equal(vel.to_numpy(), ref)
+    assert np.abs(ref).max() > 1.0
+    # a deformation solver of another size than the file is an error
+    with pytest.raises(M.FlofError):
+        M.loadVelRescaled(vel, str(tmp_path / "pair"), solver(Xd + 1, Yd, Zd, Td),
+                          dOff, dScale, dFac)

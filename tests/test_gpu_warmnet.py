@@ -109,3 +109,51 @@ def test_training_lowers_loss_and_warm_start_is_used():
         assert all(r <= 1e-5 for l in it1["levels"] for r in l["residual"])
         gains.append(sum(a0) / max(1, sum(a1)))
     say("iteration ratio plain/warm per pair: %s" % ["%.2f" % g for g in gains])
+
+
+def test_capture_and_net_level_start_bit_exact():
+    """The training pairs and the learned warm start of the finest level,
+    against the oracle: the captured pre-warped i0 is the coarse flow's
+    warp of i0 (Neumann border), the captured increment the level solve
+    from zero, and with a network the finest level's CG starts from the
+    network's guess of that increment (energy-optimal scale) -- each
+    bit-identical to the oracle with correctly rounded sums."""
+    say("capture + net level start")
+    N, T = 24, 24
+    s = M.Solver(gridSize=M.vec3(N, N, N), fourthDim=T)
+    a = O.drop_sdf(N, N, N, T, seed=11, nspheres=2, rmin=0.08, rmax=0.18)
+    b = O.drop_sdf(N, N, N, T, seed=12, nspheres=2, rmin=0.08, rmax=0.18)
+    g0, g1 = s.create(M.Grid4Real), s.create(M.Grid4Real)
+    g0.from_numpy(a)
+    g1.from_numpy(b)
+    prm = dict(WS.PLAIN, minGridSize=12)
+    p = M.default_params(**{k: int(v) if k == "doFinalProject" else v for k, v in prm.items()})
+    vel = s.create(M.Grid4Vec4)
+    err, info, i0w, inc = WS.capture(vel, g0, g1, level=0, **prm)
+    v_plain = s.create(M.Grid4Vec4)
+    err_plain = M.opticalFlowMultiscale4d(vel=v_plain, i0=g0, i1=g1, **prm)
+    assert np.array_equal(vel.to_numpy(), v_plain.to_numpy()) and err == err_plain
+    # oracle: the coarse level (12^4), prolonged, i0 warped by it
+    with O.sum_mode(1):
+        sm = (T // 2, N // 2, N // 2, N // 2)
+        a_s, b_s = O.pyramid_resample(sm, a), O.pyramid_resample(sm, b)
+        vc, _, _ = O.multiscale4d(np.zeros(sm + (4,), np.float32), a_s, b_s, p)
+        vp = O.pyramid_resample((T, N, N, N), vc) * np.float32(2.0)
+        w = O.set_bound_neumann(O.advect4d(vp, a), 0)
+        assert np.array_equal(i0w.to_numpy(), w)
+        ref_inc, it, _, _ = O.optical_flow4d(np.zeros(vp.shape, np.float32), w, b, p)
+        assert np.array_equal(inc.to_numpy(), ref_inc)
+        # the network's guess of the finest increment starts that level
+        net = WS.WarmNet(seed=21)
+        net.set_params((RNG.standard_normal(WS.num_params()) * 0.05).astype(np.float32))
+        guess = s.create(M.Grid4Vec4)
+        net.forward(i0w, g1, guess)
+        vn = s.create(M.Grid4Vec4)
+        err_n, info_n = net.solve(vn, g0, g1, levels=1, **prm)
+        ref_n, it_n, _, st_n = O.optical_flow4d(np.zeros(vp.shape, np.float32), w, b, p,
+                                                warm=guess.to_numpy())
+    sn = info_n.summary()
+    say("finest level iterations: cold", it, "net start", sn["levels"][0]["iters"][0],
+        "kept", sn["levels"][0]["warm_used"])
+    assert sn["levels"][0]["iters"][0] == it_n
+    assert np.array_equal(vn.to_numpy(), vp + ref_n)
