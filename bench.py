@@ -83,6 +83,18 @@ def main():
                     help="short workload for rocprofv3 --pmc runs of the frame "
                          "warp: a 64^4 solve capped at 50 iterations per level, "
                          "then the bench's 64-frame 128^3 warp; prints its JSON")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the sections after the frame warp (mixed precision at "
+                         "96^3 x 64, the learned warm start at 128^3 x 64, the fused "
+                         "batch, the flof.py parameter set)")
+    ap.add_argument("--flof-workload", action="store_true",
+                    help="flof.py's mode-1 parameters (multiStep 3, blur 4, final "
+                         "projection, band 0.1, wSmooth 1e-3, cgAccuracy 1e-2, "
+                         "minGridSize 20) at 32^3 x 48: GPU and the CPU oracle; prints "
+                         "its own JSON line")
+    ap.add_argument("--cpu-calibration", action="store_true",
+                    help="CPU only: the oracle's full 32^4 solve timed against the "
+                         "bounded-sample estimate the CPU baseline uses; prints JSON")
     ap.add_argument("--ops-workload", action="store_true",
                     help="the path's other operations at 64^4 (grid ops, advect4d, "
                          "interpolateGrid4d, gaussianBlur4d, calcLsDiff4d, "
@@ -94,6 +106,9 @@ def main():
               "sharded across GPUs (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.cpu_calibration:
+        print(json.dumps(cpu_calibration(args.cpu_iters)), flush=True)
+        return
     if args.no_graphs:
         os.environ["FLOF_NO_GRAPH"] = "1"  # read when the context is created
     from ofblend_amd import _lib
@@ -107,6 +122,9 @@ def main():
         return
     if args.precision_workload:
         precision_workload(M)
+        return
+    if args.flof_workload:
+        print(json.dumps(flof_workload(M, cpu=not args.no_cpu)), flush=True)
         return
     s = M.Solver(gridSize=M.vec3(N, N, N), fourthDim=N)
     ctx = M.context()
@@ -321,10 +339,16 @@ def main():
         "final_err": eb, "final_err_fp32": info["err_final"],
         "max_abs_diff_vs_fp32": float(np.abs(vel.to_numpy() - vf).max()),
         "note": "x, r, g and all arithmetic FP32, CG scalars FP64; p stored as bfloat16"}
-    out["warm_start"] = warm_start(M, s, vel, i0, i1, vf, iters, ms_step)
+    out["parity"] = parity_block(M, vcap)
 
     if not args.no_frames:
         out["frame_warp"] = frame_warp(M, L, ctx, s, vel, FLOF_KT_WARP)
+
+    if not args.no_extras:
+        out["mixed_precision"] = mixed_precision(M)
+        out["warm_start"] = warm_start_c5(M)
+        out["batch"] = fused_batch(M, L)
+        out["flof_params_32x48"] = flof_workload(M, cpu=False)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_iters, iters, cells, ms_step)
@@ -332,23 +356,64 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def precision_workload(M):
-    """Reduced-precision search direction: FP32 and BF16 solves of the same
-    2-sphere drop pair at 96^3 x 64 and 32^3 x 32 against each other (the
-    comparison with the FP64 CPU oracle is tests/study_precision.py)."""
-    from ofblend_amd.abi import FLOF_PSTORE_BF16
-    out = {"workload": "2-sphere drop pairs (seeds 7, 8; radii 0.08N-0.18N), P_plain, "
-                       "residual 1e-5"}
-    for (n, t) in ((96, 64), (32, 32)):
-        sv = M.Solver(gridSize=M.vec3(n, n, n), fourthDim=t)
-        a, b = sv.create(M.Grid4Real), sv.create(M.Grid4Real)
-        M.dropSdf4d(a, seed=7, nspheres=2, rmin=0.08, rmax=0.18)
-        M.dropSdf4d(b, seed=8, nspheres=2, rmin=0.08, rmax=0.18)
+def _golden():
+    path = os.path.join(ROOT, "tests", "golden", "digests.json")
+    return json.load(open(path)) if os.path.exists(path) else {}
+
+
+def parity_block(M, vcap):
+    """The 64^4 solve with the reference's 1000-iteration cap against the
+    oracle's golden digest (checked live: a match means the flow is the
+    oracle's bit for bit), and the recorded distances (cells) of that flow,
+    and of the 32^4 one, to the FP64 oracle and to the oracle in the
+    reference's serial summation order, for the flow and for the frame warp
+    of the same drops."""
+    import hashlib
+    d = _golden()
+    out = {}
+    if "drop64_l3_cap1000" in d:
+        out["digest_64x4_cap1000_matches"] = (
+            hashlib.sha256(vcap.tobytes()).hexdigest() == d["drop64_l3_cap1000"]["sha256"])
+    for k in ("drop32_l3", "drop64_l3_cap1000"):
+        c = d.get(k, {})
+        if "distances" not in c:
+            continue
+        out[k] = {"iters": c["iters"], "iters_fp64": c["fp64_oracle"]["iters"],
+                  "flow": c["distances"]}
+        if "warp" in c:
+            out[k]["warp_frames"] = {kk: c["warp"][kk] for kk in
+                                     ("frames", "gpu_vs_fp64", "serial_vs_fp64",
+                                      "gpu_vs_serial")}
+    out["note"] = ("gpu_vs_*: the GPU flow (= the correctly rounded oracle's, digest) "
+                   "against FP64 and against the reference's serial FP64 sums; "
+                   "serial_vs_fp64: the reference order's own distance to FP64")
+    return out
+
+
+def _drop_pair(M, n, t, seeds, nspheres, rmin=0.08, rmax=0.18):
+    sv = M.Solver(gridSize=M.vec3(n, n, n), fourthDim=t)
+    a, b = sv.create(M.Grid4Real), sv.create(M.Grid4Real)
+    M.dropSdf4d(a, seed=seeds[0], nspheres=nspheres, rmin=rmin, rmax=rmax)
+    M.dropSdf4d(b, seed=seeds[1], nspheres=nspheres, rmin=rmin, rmax=rmax)
+    return sv, a, b
+
+
+def mixed_precision(M, sizes=((96, 64),)):
+    """The mixed-precision solve (bfloat16 images, binary16 gradients of i1,
+    FP32 stencil arithmetic, FP64 CG scalars) against the FP32 solve of the
+    same 2-sphere drop pair (radii 0.08N-0.18N, seeds 7, 8), P_plain, 3
+    levels, residual 1e-5; with the 32^3 x 32 distances of both to the FP64
+    oracle recorded by tests/golden/make_digests.py (the GPU reproduces that
+    oracle's mixed-precision flow bit for bit)."""
+    from ofblend_amd.abi import FLOF_PREC_IMG_BF16
+    out = {"workload": "2-sphere drop pair (seeds 7, 8; radii 0.08N-0.18N), P_plain, "
+                       "3 levels, relative residual 1e-5"}
+    for (n, t) in sizes:
+        sv, a, b = _drop_pair(M, n, t, (7, 8), 2)
         v = sv.create(M.Grid4Vec4)
-        prm = dict(PARAMS, minGridSize=n // 4, cgMaxIter=FIXED_RESIDUAL_CAP)  # 3 levels
-        res = {}
-        flows = {}
-        for name, prec in (("fp32", 0), ("bf16", FLOF_PSTORE_BF16)):
+        prm = dict(PARAMS, minGridSize=n // 4, cgMaxIter=FIXED_RESIDUAL_CAP)
+        res, flows = {}, {}
+        for name, prec in (("fp32", 0), ("mixed", FLOF_PREC_IMG_BF16)):
             v.clear()
             M.sync()
             t0 = time.perf_counter()
@@ -357,14 +422,27 @@ def precision_workload(M):
             dt = time.perf_counter() - t0
             info = M.getLastSolveInfo().summary()
             flows[name] = v.to_numpy()
-            res[name] = {"ms": 1000 * dt, "iters_per_level": [l["iters"][0] for l in info["levels"]],
+            res[name] = {"ms": 1000 * dt,
+                         "iters_per_level": [l["iters"][0] for l in info["levels"]],
                          "final_err": e}
-        d = flows["bf16"] - flows["fp32"]
-        res["bf16_vs_fp32"] = {"max": float(np.abs(d).max()),
-                               "rms": float(np.sqrt((d ** 2).mean()))}
-        out["%dx%d" % (n, t)] = res
-        log("bench: precision %d^3 x %d done" % (n, t))
-    print(json.dumps(out), flush=True)
+        d = flows["mixed"] - flows["fp32"]
+        res["mixed_vs_fp32"] = {"max": float(np.abs(d).max()),
+                                "rms": float(np.sqrt((d ** 2).mean()))}
+        out["%dx%dx%dx%d" % (n, n, n, t)] = res
+        del flows, d, v, a, b, sv
+        log("bench: mixed precision %d^3 x %d done" % (n, t))
+    g = _golden().get("mixed32_l3")
+    if g:
+        out["32x32x32x32_vs_fp64_oracle"] = {
+            "mixed": g["distances"]["mixed_vs_fp64"], "fp32": g["distances"]["fp32_vs_fp64"],
+            "iters_mixed": g["iters"], "iters_fp32": g["iters_fp32"],
+            "iters_fp64": g["iters_fp64"], "absmax_fp64": g["absmax_fp64"]}
+    return out
+
+
+def precision_workload(M):
+    """The mixed-precision section at 96^3 x 64 and 32^3 x 32 on its own."""
+    print(json.dumps(mixed_precision(M, sizes=((96, 64), (32, 32)))), flush=True)
 
 
 def batch_workload(M, L):
@@ -451,55 +529,6 @@ def committed_traffic(kernel):
     return None
 
 
-def warm_start(M, s, vel, i0, i1, v_plain, iters_plain, ms_plain):
-    """The solve warm-started by the trained network (weights committed in
-    ofblend_amd/data) against the plain solve: iterations, time and the
-    SDF-alignment error of both (calcLsDiff, and the RMS of i0 warped by the
-    flow minus i1)."""
-    from ofblend_amd import warmstart as WS
-    if not os.path.exists(WS.DEFAULT_WEIGHTS):
-        return None
-    net = WS.WarmNet(seed=1)
-    net.load(WS.DEFAULT_WEIGHTS)
-    warm = s.create(M.Grid4Vec4)
-    res = {}
-    for rep in range(2):
-        M.sync()
-        t0 = time.perf_counter()
-        net.warm_start(i0, i1, warm, netX=32)
-        vel.clear()
-        err = M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, warm=warm,
-                                        cgMaxIter=FIXED_RESIDUAL_CAP, **PARAMS)
-        M.sync()
-        t = time.perf_counter() - t0
-    wi = M.getLastSolveInfo().summary()
-    vw = vel.to_numpy()
-    h1 = i1.to_numpy()
-
-    def rms_sdf(flow):
-        # RMS of i0 warped by the flow (advect4d) minus i1, over all cells
-        w = s.create(M.Grid4Real)
-        w.copyFrom(i0)
-        M.advect4d(flow, w)
-        d = w.to_numpy().astype(np.float64) - h1
-        return float(np.sqrt(np.mean(d * d)))
-
-    rms_warm = rms_sdf(vel)
-    vel.clear()
-    e_plain = M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, cgMaxIter=FIXED_RESIDUAL_CAP,
-                                        **PARAMS)
-    rms_plain = rms_sdf(vel)
-    it_w = [l["iters"][0] for l in wi["levels"]]
-    return {"ms_including_net": 1000.0 * t, "ms_plain": ms_plain,
-            "iters_per_level": it_w, "iters_per_level_plain": iters_plain,
-            "finest_level_iteration_ratio": iters_plain[0] / max(1, it_w[0]),
-            "warm_kept_per_level": [l["warm_used"] for l in wi["levels"]],
-            "final_err": err, "final_err_plain": e_plain,
-            "rms_sdf_diff": rms_warm, "rms_sdf_diff_plain": rms_plain,
-            "max_abs_diff_vs_plain": float(np.abs(vw - v_plain).max()),
-            "net_level": 32}
-
-
 def frame_warp(M, L, ctx, s, vel, kind):
     """The frame warp workload: 64 analytic 3D frames at 128^3 placed into a
     128^3 x 64 level set and warped by the solved 64^4 deformation (alpha 1,
@@ -531,9 +560,13 @@ def frame_warp(M, L, ctx, s, vel, kind):
     L.flof_ctx_enable_timing(ctx, kind, 0)
     kern_s = ms.value / 1000.0 / max(1, n.value)
     M3, Md = (F * N) ** 3, N ** 3
-    byts = N * (4 * M3 + 8 * M3 + 2 * 16 * Md)  # per frame: write, 2 phi slices, 2 defo slices
+    # one launch warps all N frames: it writes N frames and needs the 4D
+    # level set (N time slices of M3 cells) and the deformation (N slices of
+    # Md Vec4 cells) once each -- consecutive frames share their slices
+    byts = N * 4 * M3 + N * 4 * M3 + N * 16 * Md
     return {"frames": N, "frame": [F * N] * 3, "ms_per_64_frames": 1000 * wall,
             "frames_per_s": N / wall, "kernel_ms": 1000 * kern_s,
+            "bytes_per_launch": byts,
             "GBps_algorithmic": byts / kern_s / 1e9, "frac": byts / kern_s / HBM_PEAK,
             "place_ms": 1000 * t_place}
 
@@ -630,17 +663,178 @@ def _f4(*v):
     return (C.c_float * 4)(*v)
 
 
-def cpu_baseline(k_iters, iters, cells, gpu_ms):
-    """The CPU oracle (FP32, the reference's serial CG and FP64 dot order)
-    timed on a bounded sample: the finest-level set-up plus PCG at 64^4 with
-    an iteration cap of k and 2k; the difference gives the time per iteration,
-    scaled to a full solve with the GPU's per-level iteration counts."""
+def _rms_sdf(M, sv, flow, i0, h1):
+    """RMS of i0 warped by the flow (advect4d) minus i1, over all cells."""
+    w = sv.create(M.Grid4Real)
+    w.copyFrom(i0)
+    M.advect4d(flow, w)
+    d = w.to_numpy().astype(np.float64) - h1
+    return float(np.sqrt(np.mean(d * d)))
+
+
+def warm_start_c5(M, n=128, t=64, seeds=(9001, 9002), nspheres=2):
+    """The learned warm start at 128^3 x 64 (a drop pair the network was not
+    trained on): the network (trained on the finest-level increments of 2000
+    plain 64^4 solves, weights in ofblend_amd/data) predicts each level's
+    increment from that level's pre-warped i0 and i1; the CG refines from it
+    to the same residual (1e-5), falling back to zero where the guess's
+    residual is larger.  Against the cold solve: iterations per level, time,
+    and the SDF-alignment error of both (calcLsDiff and the RMS of i0 warped
+    by the flow minus i1)."""
+    from ofblend_amd import warmstart as WS
+    if not os.path.exists(WS.NET64_WEIGHTS):
+        return None
+    net = WS.WarmNet(seed=1)
+    net.load(WS.NET64_WEIGHTS)
+    sv, a, b = _drop_pair(M, n, t, seeds, nspheres)
+    v = sv.create(M.Grid4Vec4)
+    prm = dict(PARAMS, minGridSize=16, cgMaxIter=FIXED_RESIDUAL_CAP)
+    h1 = b.to_numpy()
+    out = {"grid": [n, n, n, t], "seeds": list(seeds), "spheres": nspheres}
+    for name in ("cold", "net_all_levels"):
+        v.clear()
+        M.sync()
+        t0 = time.perf_counter()
+        if name == "cold":
+            e = M.opticalFlowMultiscale4d(vel=v, i0=a, i1=b, **prm)
+            info = M.getLastSolveInfo().summary()
+        else:
+            e, inf = net.solve(v, a, b, levels=-1, **prm)
+            info = inf.summary()
+        M.sync()
+        dt = time.perf_counter() - t0
+        out[name] = {"ms": 1000 * dt, "iters_per_level": [l["iters"][0] for l in info["levels"]],
+                     "warm_kept_per_level": [l["warm_used"] for l in info["levels"]],
+                     "residual_per_level": [l["residual"][0] for l in info["levels"]],
+                     "final_err": e, "rms_sdf_diff": _rms_sdf(M, sv, v, a, h1)}
+        log("bench: warm start %s %.2f s iters %s" % (name, dt, out[name]["iters_per_level"]))
+    c, w = out["cold"], out["net_all_levels"]
+    out["finest_level_iteration_ratio"] = c["iters_per_level"][0] / max(1, w["iters_per_level"][0])
+    out["total_iteration_ratio"] = sum(c["iters_per_level"]) / max(1, sum(w["iters_per_level"]))
+    return out
+
+
+def fused_batch(M, L):
+    """The 16-solve batch (forward and backward of 8 seeded drop pairs of
+    1-3 spheres, N in {48,64,80}, T in {32,48,64}): one solve after another
+    against the fused batch (every PCG launch spans the levels of equal index
+    of all problems); the results must be identical."""
+    rng = np.random.default_rng(4)
+    jobs = []
+    for k in range(8):
+        n = int(rng.choice([48, 64, 80]))
+        t = int(rng.choice([32, 48, 64]))
+        sv, a, b = _drop_pair(M, n, t, (100 + 2 * k, 101 + 2 * k), 1 + k % 3)
+        for (x, y) in ((a, b), (b, a)):
+            jobs.append((sv, x, y, sv.create(M.Grid4Vec4), (n, t)))
+    prm = dict(PARAMS, cgMaxIter=FIXED_RESIDUAL_CAP)
+    M.sync()
+    t0 = time.perf_counter()
+    seq = []
+    for sv, x, y, v, _ in jobs:
+        v.clear()
+        M.opticalFlowMultiscale4d(vel=v, i0=x, i1=y, **prm)
+        seq.append(v.to_numpy())
+    M.sync()
+    t_seq = time.perf_counter() - t0
+    for _, _, _, v, _ in jobs:
+        v.clear()
+    M.sync()
+    t0 = time.perf_counter()
+    errs, infos = M.opticalFlowMultiscale4dFused([j[3] for j in jobs], [j[1] for j in jobs],
+                                                 [j[2] for j in jobs], **prm)
+    M.sync()
+    t_f = time.perf_counter() - t0
+    same = all(np.array_equal(j[3].to_numpy(), seq[q]) for q, j in enumerate(jobs))
+    log("bench: batch sequential %.2f s fused %.2f s identical %s" % (t_seq, t_f, same))
+    cells = sum(j[4][0] ** 3 * j[4][1] for j in jobs)
+    return {"workload": "16 solves: forward+backward of 8 seeded drop pairs (1-3 spheres), "
+                        "N in {48,64,80}, T in {32,48,64}, P_plain, residual 1e-5",
+            "sizes": [j[4] for j in jobs], "total_cells": cells,
+            "sequential_s": t_seq, "fused_s": t_f, "fused_solves_per_s": len(jobs) / t_f,
+            "speedup_vs_sequential": t_seq / t_f, "identical_to_sequential": same}
+
+
+FLOF_PARAMS = dict(wSmooth=0.001, wEnergy=0.0001, cgAccuracy=0.01, postVelBlur=4.0,
+                   cfl=999.0, multiStep=3, minGridSize=20, doFinalProject=True,
+                   resetBndWidth=0.1)
+
+
+def flof_workload(M, cpu=True):
+    """flof.py's mode-1 solve (scenes/flof.py:913-915: wSmooth 1e-3, wEnergy
+    1e-4, cgAccuracy 1e-2, postVelBlur 4, multiStep 3, minGridSize 20, final
+    projection, band 0.1) at its 32^3 x 48 grid, on the seeded drop pair
+    (seeds 1, 2) instead of the FLIP simulations: the GPU solve (median of 3)
+    and, with cpu, one solve of the CPU oracle in the reference's order on
+    this host's cores (the reference quotes "circa 2-3 minutes" for the whole
+    mode-1 run, README.md:35)."""
+    n, t = 32, 48
+    sv, a, b = _drop_pair(M, n, t, (1, 2), 1, rmin=0.15, rmax=0.15)
+    v = sv.create(M.Grid4Vec4)
+    times = []
+    for _ in range(4):
+        v.clear()
+        M.sync()
+        t0 = time.perf_counter()
+        e = M.opticalFlowMultiscale4d(vel=v, i0=a, i1=b, **FLOF_PARAMS)
+        M.sync()
+        times.append(time.perf_counter() - t0)
+    info = M.getLastSolveInfo().summary()
+    out = {"grid": [n, n, n, t], "params": FLOF_PARAMS,
+           "gpu_ms": 1000 * float(np.median(times[1:])), "final_err": e,
+           "iters_per_level_and_step": [l["iters"] for l in info["levels"]]}
+    if cpu:
+        from oracle import oracle as O
+        from ofblend_amd.abi import default_params
+        i0, i1 = a.to_numpy(), b.to_numpy()
+        t0 = time.perf_counter()
+        ref, rerr, _ = O.multiscale4d(np.zeros(i0.shape + (4,), np.float32), i0, i1,
+                                      default_params(**FLOF_PARAMS))
+        out["cpu_s"] = time.perf_counter() - t0
+        out["cpu_threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["nproc"] = os.cpu_count()
+        out["cpu_kind"] = "port (the oracle: serial CG, OpenMP over t in the KERNEL loops)"
+        out["speedup"] = out["cpu_s"] / (out["gpu_ms"] / 1000)
+        d = v.to_numpy() - ref
+        out["max_abs_diff_gpu_vs_reference_order"] = float(np.abs(d).max())
+        log("bench: flof.py parameters: gpu %.1f ms, cpu %.1f s" % (out["gpu_ms"], out["cpu_s"]))
+    return out
+
+
+def cpu_calibration(k_iters):
+    """The CPU baseline's bounded sample checked against a full solve: at
+    32^4 (P_plain, 3 levels) the oracle's whole solve is timed and compared
+    with the estimate from a finest-level sample of k and 2k iterations
+    scaled by the same per-level iteration counts (the method cpu_baseline
+    applies at 64^4)."""
     from oracle import oracle as O
     from ofblend_amd.abi import default_params
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    i0 = O.drop_sdf(N, N, N, N, seed=1)
-    i1 = O.drop_sdf(N, N, N, N, seed=2)
+    n = 32
+    i0 = O.drop_sdf(n, n, n, n, seed=1)
+    i1 = O.drop_sdf(n, n, n, n, seed=2)
+    t0 = time.perf_counter()
+    _, _, info = O.multiscale4d(np.zeros(i0.shape + (4,), np.float32), i0, i1,
+                                default_params(**PARAMS_C1))
+    full = time.perf_counter() - t0
+    lv = info.summary()["levels"]
+    iters = [l["iters"][0] for l in lv]
+    cells = [int(np.prod(l["dims"])) for l in lv]
+    est, tt = _cpu_estimate(i0, i1, k_iters, iters, cells)
+    return {"grid": [n] * 4, "full_solve_s": full, "estimate_s": est,
+            "estimate_over_full": est / full, "iters_per_level": iters,
+            "sample_s": sum(tt), "nproc": os.cpu_count(),
+            "omp_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+PARAMS_C1 = dict(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5, minGridSize=8, multiStep=1,
+                 postVelBlur=0.0, doFinalProject=False, resetBndWidth=-1.0, cfl=999.0)
+
+
+def _cpu_estimate(i0, i1, k_iters, iters, cells):
+    """Finest-level set-up + k and 2k PCG iterations of the oracle; the
+    difference gives the time per iteration, scaled to every level."""
+    from oracle import oracle as O
+    from ofblend_amd.abi import default_params
     v = np.zeros(i0.shape + (4,), np.float32)
     tt = []
     for k in (k_iters, 2 * k_iters):
@@ -648,19 +842,35 @@ def cpu_baseline(k_iters, iters, cells, gpu_ms):
         t0 = time.perf_counter()
         O.optical_flow4d(v, i0, i1, p)
         tt.append(time.perf_counter() - t0)
-        log("bench: cpu oracle 64^4 level solve, %d iterations: %.2f s" % (k, tt[-1]))
+        log("bench: cpu oracle %s level solve, %d iterations: %.2f s" % (i0.shape, k, tt[-1]))
     t_it = (tt[1] - tt[0]) / k_iters
     t_setup = max(0.0, tt[0] - k_iters * t_it)
     per_cell_it = t_it / cells[0]
     per_cell_setup = t_setup / cells[0]
     est = sum(i * c * per_cell_it + c * per_cell_setup for i, c in zip(iters, cells))
-    return {"value": 1.0 / est, "unit": "solves/s", "cores": threads, "kind": "port",
-            "s_per_solve": est,
-            "sample": "CPU oracle (FP32-faithful, serial CG as the reference) on "
-                      "the 64^4 finest level: set-up + %d and %d PCG iterations "
-                      "(%.1f s); time per cell-iteration %.1f ns scaled to the "
-                      "solve's per-level iteration counts %s" % (
-                          k_iters, 2 * k_iters, sum(tt), per_cell_it * 1e9, iters)}
+    return est, tt
+
+
+def cpu_baseline(k_iters, iters, cells, gpu_ms):
+    """The CPU oracle (FP32, the reference's serial CG and FP64 dot order)
+    timed on a bounded sample: the finest-level set-up plus PCG at 64^4 with
+    an iteration cap of k and 2k; the difference gives the time per iteration,
+    scaled to a full solve with the GPU's per-level iteration counts (the
+    same estimate matches a full 32^4 solve: bench.py --cpu-calibration).
+    The CG is serial like the reference's; the set-up loops use OpenMP."""
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    i0 = O.drop_sdf(N, N, N, N, seed=1)
+    i1 = O.drop_sdf(N, N, N, N, seed=2)
+    est, tt = _cpu_estimate(i0, i1, k_iters, iters, cells)
+    return {"value": 1.0 / est, "unit": "solves/s", "cores": 1, "kind": "port",
+            "s_per_solve": est, "nproc": os.cpu_count(),
+            "omp_threads_setup": int(os.environ["OMP_NUM_THREADS"]),
+            "sample": "CPU oracle (FP32-faithful, serial CG as the reference: one core "
+                      "does the PCG) on the 64^4 finest level: set-up + %d and %d PCG "
+                      "iterations (%.1f s), scaled to the solve's per-level iteration "
+                      "counts %s" % (k_iters, 2 * k_iters, sum(tt), iters)}
 
 
 if __name__ == "__main__":

@@ -176,6 +176,27 @@ enum flof_stencil {
 };
 int flof_ctx_set_stencil(flof_ctx *ctx, int kind);
 
+/* ---- level-set mesh of a 3D frame and its .bobj.gz file ---------------- */
+typedef struct flof_mesh flof_mesh;
+/* LevelsetGrid::createMesh (source/levelset.cpp:266-365) of a 3D level set
+ * phi (a float[Z][Y][X] device buffer, X, Y, Z >= 2): marching cubes on the
+ * iso value 1e-4 of -phi, cubes with a value at or below -1000 skipped; the
+ * nodes and triangles are numbered as the reference numbers them.  The
+ * mesh's normals are those its file holds (Mesh::computeVertexNormals,
+ * source/mesh.cpp:460-479); the mesher's own gradient normals, which
+ * writeBobjFile replaces, are not formed. */
+int flof_create_mesh(flof_ctx *ctx, const float *phi_dev, int X, int Y, int Z,
+                     flof_mesh **out);
+int flof_mesh_size(const flof_mesh *m, int64_t *nodes, int64_t *tris);
+/* Host copies: pos and normals 3 floats per node, tris 3 ints per triangle
+ * (any pointer may be NULL). */
+int flof_mesh_download(const flof_mesh *m, float *pos, float *normals, int32_t *tris);
+/* writeBobjFile (source/fileio.cpp:121-160): gzip, int nodes, positions
+ * (p - gs/2) / max(gs) for the level set's grid size gs, int nodes,
+ * normals, int triangles, 3 ints each. */
+int flof_mesh_save_bobj(const flof_mesh *m, const char *name);
+int flof_mesh_destroy(flof_mesh *m);
+
 /* ---- Output post-processing of warped 3D frames (scenes/flof.py:737-782,
  * scenes/ofHelpers.py:108-115).  A frame is a float[Z][Y][X] device buffer;
  * Z == 1 is a 2D grid. */

@@ -49,6 +49,11 @@ SIGNATURES = {
     "flof_ctx_set_stencil": ([_P, _I], _I),
     "flof_stream_bandwidth": ([_P, _I, C.c_int64, _I, _DP], _I),
     "flof_check_reciprocal": ([_P, C.c_uint32, C.c_uint32, C.POINTER(_U64)], _I),
+    "flof_create_mesh": ([_P, _P, _I, _I, _I, C.POINTER(_P)], _I),
+    "flof_mesh_size": ([_P, _I64P, _I64P], _I),
+    "flof_mesh_download": ([_P, _P, _P, _P], _I),
+    "flof_mesh_save_bobj": ([_P, C.c_char_p], _I),
+    "flof_mesh_destroy": ([_P], _I),
     "flof_frame_op": ([_P, _I, _P, _P, C.c_int64, _F], _I),
     "flof_simple_blur_special": ([_P, _P, _I, _I, _I, _I, _F, _I], _I),
     "flof_frame_set_bound": ([_P, _P, _I, _I, _I, _F, _I], _I),

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build", "flof_hip")
 LIB = os.path.join(HERE, "libflof_hip.so")
 SOURCES = ["flof_core.hip", "flof_solve.hip", "flof_frames.hip", "flof_prep.hip",
-           "flof_warmnet.hip", "flof_post.hip"]
+           "flof_warmnet.hip", "flof_post.hip", "flof_mesh.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

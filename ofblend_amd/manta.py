@@ -468,6 +468,15 @@ class DeviceArray:
                                               int(boundaryWidth)), self.ctx)
         return self
 
+    def createMesh(self, mesh, notiming=False):
+        """LevelsetGrid::createMesh (source/levelset.cpp:266-365) of this 3D
+        level set into `mesh` (a Mesh), on the GPU."""
+        z, y, x = self.shape
+        h = C.c_void_p()
+        check(_lib.lib().flof_create_mesh(self.ctx, self.ptr, x, y, z, C.byref(h)), self.ctx)
+        mesh._set(h, self.ctx)
+        return mesh
+
     def __del__(self):
         if getattr(self, "ptr", None) is not None and self.ptr.value:
             try:
@@ -475,6 +484,64 @@ class DeviceArray:
             except Exception:
                 pass
             self.ptr = None
+
+
+class Mesh:
+    """A triangle mesh made by createMesh (source/mesh.h): nodes, their file
+    normals and triangles, kept on the GPU; save() writes .bobj.gz files
+    (writeBobjFile, source/fileio.cpp:121-160)."""
+
+    def __init__(self, parent=None):
+        self.h = None
+        self.ctx = None
+
+    def _set(self, h, ctx):
+        self._free()
+        self.h, self.ctx = h, ctx
+
+    def _free(self):
+        if self.h is not None and self.h.value:
+            try:
+                _lib.lib().flof_mesh_destroy(self.h)
+            except Exception:
+                pass
+        self.h = None
+
+    def __del__(self):
+        self._free()
+
+    def _size(self):
+        if self.h is None:
+            return 0, 0
+        nv, nt = C.c_int64(), C.c_int64()
+        _lib.lib().flof_mesh_size(self.h, C.byref(nv), C.byref(nt))
+        return nv.value, nt.value
+
+    def numNodes(self):
+        return self._size()[0]
+
+    def numTris(self):
+        return self._size()[1]
+
+    def to_numpy(self):
+        """(positions (V, 3), normals (V, 3), triangles (F, 3))."""
+        nv, nt = self._size()
+        pos = np.empty((nv, 3), np.float32)
+        nrm = np.empty((nv, 3), np.float32)
+        tri = np.empty((nt, 3), np.int32)
+        if self.h is not None:
+            check(_lib.lib().flof_mesh_download(self.h, pos.ctypes.data_as(C.c_void_p),
+                                                nrm.ctypes.data_as(C.c_void_p),
+                                                tri.ctypes.data_as(C.c_void_p)), self.ctx)
+        return pos, nrm, tri
+
+    def save(self, name, notiming=False):
+        """Mesh::save for .bobj.gz files."""
+        if not str(name).endswith(".bobj.gz"):
+            raise FlofError(-1, "mesh files: only .bobj.gz is written")
+        if self.h is None:
+            raise FlofError(-1, "mesh: empty (no createMesh yet)")
+        check(_lib.lib().flof_mesh_save_bobj(self.h, str(name).encode()), self.ctx)
 
 
 def getSliceFrom4d(src, srct, dst):

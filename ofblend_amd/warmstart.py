@@ -171,3 +171,7 @@ def train(net, data, steps, lr=2e-3, seed=0, log=None, lr_end=None):
 
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                "warmnet_32.npy")
+# the network trained on the finest-level increments of 2000 plain 64^4
+# solves (python -m ofblend_amd.train_warmnet), used by WarmNet.solve
+NET64_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                             "warmnet_64.npy")

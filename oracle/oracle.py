@@ -82,6 +82,9 @@ def _declare(L):
     L.orc_set_sum_mode.restype = None
     L.orc_get_sum_mode.argtypes = []
     L.orc_get_sum_mode.restype = C.c_int
+    L.orc_create_mesh.argtypes = [_P, _I, _I, _I, _P, _P, _P, _I64, _I64,
+                                  C.POINTER(_I64), C.POINTER(_I64)]
+    L.orc_create_mesh.restype = _I
     for name in ("orc_round_bf16", "orc_round_f16"):
         getattr(L, name).argtypes = [C.c_float]
         getattr(L, name).restype = C.c_float
@@ -424,6 +427,26 @@ def extrap4d_ls_simple(phi, distance=4, inside=False):
     g = np.array(phi, copy=True, order="C")
     _fn("orc_extrap4d_ls_simple", g)(_ptr(g), *_dims(g), distance, int(inside))
     return g
+
+
+def create_mesh(phi):
+    """LevelsetGrid::createMesh (levelset.cpp:266-365) of a 3D level set
+    (Z, Y, X) float32, with the vertex normals of Mesh::computeVertexNormals
+    (mesh.cpp:460-479): returns (positions (V, 3), normals (V, 3),
+    triangles (F, 3))."""
+    phi = np.ascontiguousarray(phi, dtype=np.float32)
+    Z, Y, X = phi.shape
+    nv, nt = C.c_int64(), C.c_int64()
+    cap_v, cap_t = 1 << 16, 1 << 17
+    while True:
+        pos = np.empty((cap_v, 3), np.float32)
+        nrm = np.empty((cap_v, 3), np.float32)
+        tri = np.empty((cap_t, 3), np.int32)
+        rc = lib().orc_create_mesh(_ptr(phi), X, Y, Z, _ptr(pos), _ptr(nrm), _ptr(tri),
+                                   cap_v, cap_t, C.byref(nv), C.byref(nt))
+        if rc == 0:
+            return pos[:nv.value].copy(), nrm[:nv.value].copy(), tri[:nt.value].copy()
+        cap_v, cap_t = max(cap_v, nv.value), max(cap_t, nt.value)
 
 
 P_PLAIN = dict(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5, multiStep=1,
