@@ -126,6 +126,8 @@ typedef struct flof_solve_info {
 void flof_of_params_default(flof_of_params *p);
 int flof_ctx_create(flof_ctx **out, int device);
 int flof_ctx_destroy(flof_ctx *ctx);
+/* Message of the last failure of ctx; with ctx == NULL, of the last failed
+ * call without a context (flof_calc_ls_diff4d_noalloc) on this thread. */
 const char *flof_last_error(const flof_ctx *ctx);
 int flof_sync(flof_ctx *ctx);
 /* HIP stream (hipStream_t) the context enqueues on. */
@@ -320,6 +322,22 @@ int flof_advect_cfl4d(flof_grid4d *vel, flof_grid4d *grid, float cfl,
 /* ---- error metric (optflow4d.cpp:895-927,2132-2137) -------------------- */
 int flof_calc_ls_diff4d(flof_grid4d *i0, flof_grid4d *i1, flof_grid4d *out,
                         float correction, int bnd, float *result);
+
+/* calcLsDiff4d without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels as flof_calc_ls_diff4d on plain device pointers (grids of X*Y*Z*T
+ * floats, x fastest; out may be NULL), enqueued on `stream` (a hipStream_t
+ * of the current device, NULL: the null stream) without a host sync: the
+ * reference's Real result is stored to the device float *result_dev.  The
+ * workgroup partials go to a fixed per-device scratch array of 2048
+ * double-double partials (32 KiB, held by each GPU until the process ends),
+ * so calls that overlap on one GPU (on different streams) race.  Grids of
+ * 2^31 cells or more are rejected (FLOF_ERR_RANGE); errors are reported by
+ * flof_last_error(NULL) on the calling thread. */
+int flof_calc_ls_diff4d_noalloc(const float *i0, const float *i1, float *out, int X,
+                                int Y, int Z, int T, float correction, int bnd,
+                                void *stream, float *result_dev);
 
 /* ---- blur (optflow4d.cpp:127-174) -------------------------------------- */
 int flof_gaussian_blur4d(flof_grid4d *grid, float sigma, int iter);

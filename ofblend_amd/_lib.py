@@ -88,6 +88,7 @@ SIGNATURES = {
     "flof_advect4d": ([_P, _P, _F, _F], _I),
     "flof_advect_cfl4d": ([_P, _P, _F, _F, _F], _I),
     "flof_calc_ls_diff4d": ([_P, _P, _P, _F, _I, _FP], _I),
+    "flof_calc_ls_diff4d_noalloc": ([_P, _P, _P, _I, _I, _I, _I, _F, _I, _P, _P], _I),
     "flof_gaussian_blur4d": ([_P, _F, _I], _I),
     "flof_optical_flow4d": ([_P, _P, _P, _P, C.POINTER(OfParams),
                              C.POINTER(SolveInfo)], _I),

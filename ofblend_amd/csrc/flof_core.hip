@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <new>
 
 #include "flof_internal.h"
@@ -842,6 +843,91 @@ __global__ void k_drops(float *out, int fx, int fy, int fz, int T, int Y, double
     }
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free calcLsDiff.  Its workgroup partials live in `scratch`, a
+// file-scope device array of NOALLOC_PARTS double-doubles (32 KiB) that every
+// GPU the library runs on holds until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is a stand-in symbol,
+// not a device address): noalloc_scratch() looks up each device's own
+// address once with hipGetSymbolAddress, while that device is current.
+// Every workgroup writes its own partial before the finalizer reads it, so
+// nothing needs clearing; two calls that overlap on one GPU (different
+// streams) share the array and race.
+static const int NOALLOC_PARTS = 2048;
+static __device__ __align__(256) DD scratch[NOALLOC_PARTS];
+static const int NOALLOC_MAX_DEVICES = 64;
+
+struct NoallocDev {
+    DD *part = nullptr;
+    int num_cu = 0;
+};
+static std::mutex g_noalloc_mu;
+static NoallocDev g_noalloc[NOALLOC_MAX_DEVICES];
+
+static NoallocDev noalloc_scratch()
+{
+    int dev = -1;
+    FLOF_HIP(hipGetDevice(&dev));
+    FLOF_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, FLOF_ERR_RANGE,
+                 "calcLsDiff4d (no allocation): device index above 63");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoallocDev &d = g_noalloc[dev];
+    if (!d.part) {
+        void *p = nullptr;
+        FLOF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        FLOF_REQUIRE(p != nullptr, FLOF_ERR_HIP,
+                     "calcLsDiff4d (no allocation): no device address for the scratch array");
+        int cu = 0;
+        FLOF_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+        d.num_cu = cu > 0 ? cu : 256;
+        d.part = (DD *)p;
+    }
+    return d;
+}
+
+// finalizer of the allocation-free form: the sum of the partials scaled as
+// ls_scale() does on the host, stored as the reference's Real
+__global__ void k_dd_fin_ls(const DD *part, int nb, int four_d, double inv_cells,
+                            float *result)
+{
+    DD r{0., 0.};
+    for (int bb = threadIdx.x; bb < nb; bb += blockDim.x) dd_merge(r, part[bb]);
+    r = block_dd(r);
+    if (threadIdx.x == 0) {
+        double accu = r.hi + r.lo;
+        accu *= 1000.;
+        if (four_d) accu *= 1000.;
+        accu *= inv_cells;
+        result[0] = (float)accu;
+    }
+}
+
+static void noalloc_ls_diff(const float *a, const float *b, float *out, const Dims4 &D,
+                            float corr, int bnd, hipStream_t stream, float *result_dev)
+{
+    const NoallocDev nd = noalloc_scratch();
+    const int64_t n = (int64_t)D.st * D.T;
+    // the workgroup count of the context form (8 per CU, a multiple of 8)
+    int64_t nb = (n + 255) / 256;
+    const int64_t cap = (int64_t)nd.num_cu * 8;
+    if (nb > cap) nb = cap;
+    if (nb < 1) nb = 1;
+    nb = ((nb + FLOF_NXCD - 1) / FLOF_NXCD) * FLOF_NXCD;
+    FLOF_REQUIRE(nb <= NOALLOC_PARTS, FLOF_ERR_RANGE,
+                 "calcLsDiff4d (no allocation): more workgroups than scratch partials");
+    const int64_t sx = D.X - 2 * bnd, sy = D.Y - 2 * bnd, sz = D.Z - 2 * bnd,
+                  st = D.T - 2 * bnd;
+    const double inv_cells = 1. / double(sx * sy * sz * st);
+    hipLaunchKernelGGL(k_lsdiff<float>, dim3((unsigned)nb), dim3(256), 0, stream, a, b, out,
+                       D, make_coord4(D), corr, bnd, nd.part);
+    check_launch("k_lsdiff (no allocation)");
+    hipLaunchKernelGGL(k_dd_fin_ls, dim3(1), dim3(256), 0, stream, (const DD *)nd.part,
+                       (int)nb, D.T > 1 ? 1 : 0, inv_cells, result_dev);
+    check_launch("k_dd_fin_ls");
+}
+
+// message of the last failed call that has no context, per host thread
+thread_local std::string g_noctx_err;
 
 }  // namespace flof
 
@@ -1012,7 +1098,8 @@ int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T)
 
 const char *flof_last_error(const flof_ctx *ctx)
 {
-    return ctx ? ctx->err.c_str() : "null context";
+    if (ctx) return ctx->err.c_str();
+    return flof::g_noctx_err.empty() ? "null context" : flof::g_noctx_err.c_str();
 }
 
 int flof_sync(flof_ctx *ctx)
@@ -1622,6 +1709,30 @@ int flof_calc_ls_diff4d(flof_grid4d *i0, flof_grid4d *i1, flof_grid4d *out,
                                   out ? (float *)out->data : nullptr, dims_of(i0),
                                   correction, bnd);
     API_CATCH(ctx)
+}
+
+int flof_calc_ls_diff4d_noalloc(const float *i0, const float *i1, float *out, int X,
+                                int Y, int Z, int T, float correction, int bnd,
+                                void *stream, float *result_dev)
+{
+    try {
+        FLOF_REQUIRE(i0 && i1 && result_dev, FLOF_ERR_ARG,
+                     "calcLsDiff4d (no allocation): null argument");
+        FLOF_REQUIRE(X > 0 && Y > 0 && Z > 0 && T > 0, FLOF_ERR_DIM,
+                     "calcLsDiff4d (no allocation): sizes must be positive");
+        FLOF_REQUIRE((int64_t)X * Y * Z * T < (1ll << 31), FLOF_ERR_RANGE,
+                     "calcLsDiff4d (no allocation): more than 2^31 - 1 cells");
+        FLOF_REQUIRE(bnd >= 0, FLOF_ERR_ARG, "calcLsDiff4d (no allocation): negative bnd");
+        const Dims4 D{X, Y, Z, T, (int64_t)X * Y, (int64_t)X * Y * Z};
+        noalloc_ls_diff(i0, i1, out, D, correction, bnd, (hipStream_t)stream, result_dev);
+    } catch (const flof::Error &e) {
+        flof::g_noctx_err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        flof::g_noctx_err = e.what();
+        return FLOF_ERR_ARG;
+    }
+    return FLOF_OK;
 }
 
 int flof_drop_sdf4d(flof_grid4d *phi, uint64_t seed, int nspheres, double rmin,

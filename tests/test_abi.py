@@ -52,6 +52,9 @@ def test_errors_are_reported_not_thrown():
     assert L.flof_grid4d_create(None, 4, 4, 4, 4, 1, 0, None) != 0
     assert L.flof_sync(None) != 0
     assert L.flof_last_error(None) == b"null context"
+    # the allocation-free entry point checks its arguments before any HIP call
+    rc = L.flof_calc_ls_diff4d_noalloc(None, None, None, 4, 4, 4, 4, 1.0, 0, None, None)
+    assert rc == -1 and b"null argument" in L.flof_last_error(None)
 
 
 def test_bench_refuses_more_than_one_gpu():
