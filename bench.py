@@ -728,6 +728,8 @@ def fused_batch(M, L):
         for (x, y) in ((a, b), (b, a)):
             jobs.append((sv, x, y, sv.create(M.Grid4Vec4), (n, t)))
     prm = dict(PARAMS, cgMaxIter=FIXED_RESIDUAL_CAP)
+    # launch shapes per grid shape from the autotuner (both runs)
+    L.flof_ctx_set_autotune(M.context(), 1)
     M.sync()
     t0 = time.perf_counter()
     seq = []
@@ -745,6 +747,14 @@ def fused_batch(M, L):
                                                  [j[2] for j in jobs], **prm)
     M.sync()
     t_f = time.perf_counter() - t0
+    tuned = {}
+    for n, t in sorted(set(j[4] for j in jobs)):
+        tb, lb, sl = C.c_int(), C.c_int(), C.c_int()
+        if L.flof_ctx_tuned_shape(M.context(), n, n, n, t, C.byref(tb), C.byref(lb),
+                                  C.byref(sl)) == 0:
+            tuned["%dx%d" % (n, t)] = {"tmarch_wg_per_cu": tb.value,
+                                       "lds_items_per_cu": lb.value, "lds_t_segment": sl.value}
+    L.flof_ctx_set_autotune(M.context(), 0)
     same = all(np.array_equal(j[3].to_numpy(), seq[q]) for q, j in enumerate(jobs))
     log("bench: batch sequential %.2f s fused %.2f s identical %s" % (t_seq, t_f, same))
     cells = sum(j[4][0] ** 3 * j[4][1] for j in jobs)
@@ -752,7 +762,8 @@ def fused_batch(M, L):
                         "N in {48,64,80}, T in {32,48,64}, P_plain, residual 1e-5",
             "sizes": [j[4] for j in jobs], "total_cells": cells,
             "sequential_s": t_seq, "fused_s": t_f, "fused_solves_per_s": len(jobs) / t_f,
-            "speedup_vs_sequential": t_seq / t_f, "identical_to_sequential": same}
+            "speedup_vs_sequential": t_seq / t_f, "identical_to_sequential": same,
+            "autotuned_finest_shapes": tuned}
 
 
 FLOF_PARAMS = dict(wSmooth=0.001, wEnergy=0.0001, cgAccuracy=0.01, postVelBlur=4.0,

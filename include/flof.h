@@ -154,12 +154,21 @@ int flof_ctx_timing_tag(flof_ctx *ctx, int kind, int64_t tag, int64_t *launches,
                         double *ms_total);
 
 /* Autotuner of the PCG kernels' launch shape (no reference counterpart):
- * while enabled, the first solve of each grid shape times the candidate
- * workgroups-per-CU counts of the t-marching kernels and keeps the fastest
- * for that shape (results do not depend on it).  flof_ctx_tuned_blocks
- * returns the choice for a shape, or 0 if it has not been tuned. */
+ * while enabled, the first solve of each grid shape times one PCG iteration
+ * for each candidate -- the t-marching kernels at 2, 4, 8 and 16 workgroups
+ * per CU, and, where the LDS-tiled stencil kernels apply (FP32 images, the
+ * stencil choice not forced to the t-marching kernels, grids below 2^31
+ * cells), their 64x4x4-cell tiles with t segments of the lengths that 1, 2,
+ * 4 and 8 work items per CU give -- and keeps the fastest for that shape
+ * (results do not depend on it).  flof_ctx_tuned_blocks returns the
+ * t-marching workgroups per CU for a shape, or 0 if it has not been tuned;
+ * flof_ctx_tuned_shape returns 1 if not tuned, else FLOF_OK and the choice:
+ * t-marching workgroups per CU, LDS work items per CU (0: the t-marching
+ * stencil kernels) and the LDS tiles' t-segment length (0 likewise). */
 int flof_ctx_set_autotune(flof_ctx *ctx, int enable);
 int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T);
+int flof_ctx_tuned_shape(const flof_ctx *ctx, int X, int Y, int Z, int T, int *tm_bpc,
+                         int *lds_bpc, int *lds_seglen);
 
 /* Kernels of the PCG's two stencil passes, p.(A p) and the update (no
  * reference counterpart; results are identical): FLOF_STENCIL_LDS tiles

@@ -31,6 +31,7 @@ _U64 = C.c_uint64
 _FP = C.POINTER(C.c_float)
 _DP = C.POINTER(C.c_double)
 _I64P = C.POINTER(C.c_int64)
+_IP = C.POINTER(C.c_int)
 
 SIGNATURES = {
     "flof_of_params_default": ([C.POINTER(OfParams)], None),
@@ -46,6 +47,7 @@ SIGNATURES = {
     "flof_ctx_timing_tag": ([_P, _I, C.c_int64, _I64P, _DP], _I),
     "flof_ctx_set_autotune": ([_P, _I], _I),
     "flof_ctx_tuned_blocks": ([_P, _I, _I, _I, _I], _I),
+    "flof_ctx_tuned_shape": ([_P, _I, _I, _I, _I, _IP, _IP, _IP], _I),
     "flof_ctx_set_stencil": ([_P, _I], _I),
     "flof_stream_bandwidth": ([_P, _I, C.c_int64, _I, _DP], _I),
     "flof_check_reciprocal": ([_P, C.c_uint32, C.c_uint32, C.POINTER(_U64)], _I),

@@ -1093,7 +1093,26 @@ int flof_ctx_tuned_blocks(const flof_ctx *ctx, int X, int Y, int Z, int T)
     if (!ctx) return FLOF_ERR_ARG;
     auto it = ctx->tm_tuned.find(
         std::array<int, 5>{X, Y, Z, T, tune_variant(ctx)});
-    return it == ctx->tm_tuned.end() ? 0 : it->second;
+    return it == ctx->tm_tuned.end() ? 0 : it->second.bpc;
+}
+
+int flof_ctx_tuned_shape(const flof_ctx *ctx, int X, int Y, int Z, int T, int *tm_bpc,
+                         int *lds_bpc, int *lds_seglen)
+{
+    if (!ctx) return FLOF_ERR_ARG;
+    auto it = ctx->tm_tuned.find(
+        std::array<int, 5>{X, Y, Z, T, tune_variant(ctx)});
+    if (it == ctx->tm_tuned.end()) return 1;
+    if (tm_bpc) *tm_bpc = it->second.bpc;
+    if (lds_bpc) *lds_bpc = it->second.lds_bpc;
+    if (lds_seglen) {
+        *lds_seglen = 0;
+        if (it->second.lds_bpc > 0) {
+            const Dims4 D{X, Y, Z, T, (int64_t)X * Y, (int64_t)X * Y * Z};
+            *lds_seglen = flof::lds_seglen(ctx, D, it->second.lds_bpc);
+        }
+    }
+    return FLOF_OK;
 }
 
 const char *flof_last_error(const flof_ctx *ctx)

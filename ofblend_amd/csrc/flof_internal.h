@@ -98,9 +98,15 @@ struct flof_ctx {
     int64_t timer_tag = 0;
     // worker contexts of batched solves (same device, own streams)
     std::vector<flof_ctx *> workers;
-    // t-march autotuner: on/off and the chosen workgroups per CU per shape
+    // launch-shape autotuner: on/off and the choice per grid shape
     bool autotune = false;
-    std::map<std::array<int, 5>, int> tm_tuned;  // X, Y, Z, T, p storage
+    struct TunedShape {
+        int bpc;      // workgroups per CU of the t-marching kernels
+        int lds_bpc;  // work items per CU of the LDS-tiled stencil kernels
+                      // (sets their t-segment length); 0: the t-marching
+                      // stencil kernels
+    };
+    std::map<std::array<int, 5>, TunedShape> tm_tuned;  // X, Y, Z, T, variant
     // storage of the CG search direction for the call in progress
     // (FLOF_PSTORE_*), set from flof_of_params.precision by the entry points
     int p_store = FLOF_PSTORE_F32;
@@ -601,5 +607,7 @@ double host_ls_diff_bf16(flof_ctx *ctx, const void *a, const void *b, const Dims
 // (the network must belong to ctx); keeps no activations for a backward pass
 void warmnet_infer(flof_ctx *ctx, flof_warmnet *net, const float *i0, const float *i1,
                    const Dims4 &D, float4 *flow);
+// t-segment length of the LDS-tiled stencil kernels at lds_bpc work items per CU
+int lds_seglen(const flof_ctx *ctx, const Dims4 &D, int lds_bpc);
 
 }  // namespace flof

@@ -1833,7 +1833,7 @@ static TMarch make_tmarch(flof_ctx *ctx, const Dims4 &D, int blocks_per_cu)
     return M;
 }
 
-static TileMarch make_tilemarch(flof_ctx *ctx, const Dims4 &D, int blocks_per_cu)
+static TileMarch make_tilemarch(const flof_ctx *ctx, const Dims4 &D, int blocks_per_cu)
 {
     TileMarch M;
     M.ntx = (D.X + LT_X - 1) / LT_X;
@@ -1851,8 +1851,15 @@ static TileMarch make_tilemarch(flof_ctx *ctx, const Dims4 &D, int blocks_per_cu
     return M;
 }
 
-// candidates of the t-march autotuner (workgroups per CU)
+int lds_seglen(const flof_ctx *ctx, const Dims4 &D, int lds_bpc)
+{
+    return make_tilemarch(ctx, D, lds_bpc).seglen;
+}
+
+// candidates of the autotuner: workgroups per CU of the t-marching kernels
+// and work items per CU of the LDS-tiled stencil kernels
 static const int TM_CANDIDATES[] = {2, 4, 8, 16};
+static const int LDS_CANDIDATES[] = {1, 2, 4, 8};
 
 struct LevelWork {
     Tmp g, prec, x, r, p, q, st, part;
@@ -1862,13 +1869,14 @@ struct LevelWork {
     int cap = 0;    // partial slots
     TMarch tm;
     TileMarch tl;
-    void set_tm(flof_ctx *ctx, const Dims4 &D, int bpc)
+    // lds_bpc: work items per CU of the LDS-tiled kernels (<= 0: the
+    // context's default)
+    void set_tm(flof_ctx *ctx, const Dims4 &D, int bpc, int lds_bpc = 0)
     {
         tm = make_tmarch(ctx, D, bpc);
         nb_tm = ((tm.total + FLOF_NXCD - 1) / FLOF_NXCD) * FLOF_NXCD;
-        tl = make_tilemarch(ctx, D, ctx->lds_blocks_per_cu);
+        tl = make_tilemarch(ctx, D, lds_bpc > 0 ? lds_bpc : ctx->lds_blocks_per_cu);
         nb_lds = tl.xcd ? ((tl.total + FLOF_NXCD - 1) / FLOF_NXCD) * FLOF_NXCD : tl.total;
-        // the LDS kernels index cells (and the t+2 prefetch) in 32 bits
         // the LDS tiles pay off on long t segments of full-width tiles with
         // FP32 direction storage (measured: 3% faster at 64^4, 35% slower
         // at 32^4 with 4-step segments, 26% slower with bfloat16 storage);
@@ -1876,10 +1884,12 @@ struct LevelWork {
         const bool fits = D.st * (int64_t)(D.T + 2) < (1ll << 31);
         const bool pays = ctx->p_store == FLOF_PSTORE_F32 && tl.seglen >= 16 &&
                           10 * D.X >= 9 * LT_X * tl.ntx;
-        lds = fits && !ctx->img_bf16 &&
+        lds_allowed = fits && !ctx->img_bf16 && ctx->stencil != FLOF_STENCIL_TMARCH;
+        lds = lds_allowed &&
               (ctx->stencil == FLOF_STENCIL_LDS || (ctx->stencil == FLOF_STENCIL_AUTO && pays));
     }
-    bool lds = false;  // p.Ap and the update run as the LDS-tiled kernels
+    bool lds = false;          // p.Ap and the update run as the LDS-tiled kernels
+    bool lds_allowed = false;  // the LDS-tiled kernels may run this level
     int nb_stencil() const { return lds ? nb_lds : nb_tm; }
     LevelWork(flof_ctx *ctx, const Dims4 &D)
     {
@@ -1892,8 +1902,8 @@ struct LevelWork {
         p = Tmp(ctx, b);
         q = Tmp(ctx, b);
         nb = grid_blocks_xcd(ctx, n);
-        // partial slots for the largest t-march grid the autotuner may pick
-        set_tm(ctx, D, TM_CANDIDATES[3]);
+        // partial slots for the largest grids the autotuner may pick
+        set_tm(ctx, D, TM_CANDIDATES[3], LDS_CANDIDATES[3]);
         cap = std::max(nb, std::max(nb_tm, nb_lds));
         set_tm(ctx, D, ctx->tm_blocks_per_cu);
         cap = std::max(cap, std::max(nb_tm, nb_lds));
@@ -1988,16 +1998,21 @@ static void enqueue_tm_iter_any(flof_ctx *ctx, int pstore, LevelWork &W, void *p
                                            kd, finalizers);
 }
 
-// Autotuner of the t-march decomposition: on the first solve of a grid
-// shape, times one PCG iteration (the three stencil kernels, twice) for
-// each candidate number of workgroups per CU on the level's own (not yet
-// initialised) workspace, with a scratch solver state that lets every kernel
-// run, and keeps the fastest per shape for the context's lifetime.  The
-// workspace is written by the set-up kernel afterwards, so the result of the
-// solve does not depend on the tuning.
-static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const void *i1,
-                     const LevelGeom &G, float dxf, int pstore, void *const pb[2],
-                     bool fused = false)
+// Autotuner of the launch shape: on the first solve of a grid shape, times
+// one PCG iteration (the stencil and direction kernels, twice) for each
+// candidate on the level's own (not yet initialised) workspace, with a
+// scratch solver state that lets every kernel run, and keeps the fastest
+// per shape for the context's lifetime.  The candidates are the t-marching
+// kernels' workgroups per CU and then, where the LDS-tiled stencil kernels
+// may run, their work items per CU (which fix the t-segment length of the
+// 64x4x4-cell tiles; counts that give the same segments are timed once)
+// with the best t-marching direction kernel.  Every candidate computes the
+// same values (correctly rounded sums, order-free maxima), and the
+// workspace is written by the set-up kernel afterwards, so the result of
+// the solve does not depend on the tuning.
+static flof_ctx::TunedShape tuned_shape(flof_ctx *ctx, const Dims4 &D, LevelWork &W,
+                                        const void *i1, const LevelGeom &G, float dxf,
+                                        int pstore, void *const pb[2], bool fused = false)
 {
     const std::array<int, 5> key{D.X, D.Y, D.Z, D.T, tune_variant(ctx) + (fused ? 64 : 0)};
     auto it = ctx->tm_tuned.find(key);
@@ -2014,29 +2029,51 @@ static int tuned_bpc(flof_ctx *ctx, const Dims4 &D, LevelWork &W, const void *i1
     FLOF_HIP(hipEventCreate(&e0));
     FLOF_HIP(hipEventCreate(&e1));
     CgState *st = stt.as<CgState>();
-    auto iterate = [&]() {
+    hipError_t err = hipSuccess;
+    // milliseconds of two iterations after a warm-up one (< 0: error)
+    auto time_it = [&]() -> float {
         enqueue_tm_iter_any(ctx, pstore, W, pb[0], pb[1], false, i1, G, dxf, st, -1, -1, -1,
                             false);
-    };
-    int best = ctx->tm_blocks_per_cu;
-    float best_ms = 1e30f;
-    hipError_t err = hipSuccess;
-    for (int bpc : TM_CANDIDATES) {
-        W.set_tm(ctx, D, bpc);
-        if (fused) W.lds = false;
-        if (W.nb_tm > W.cap || W.nb_lds > W.cap) continue;
-        iterate();  // warm-up
         err = hipEventRecord(e0, ctx->stream);
-        iterate();
-        iterate();
+        for (int k = 0; k < 2; ++k)
+            enqueue_tm_iter_any(ctx, pstore, W, pb[0], pb[1], false, i1, G, dxf, st, -1, -1,
+                                -1, false);
         if (err == hipSuccess) err = hipEventRecord(e1, ctx->stream);
         if (err == hipSuccess) err = hipEventSynchronize(e1);
         float ms = 0.f;
         if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-        if (err != hipSuccess) break;
+        return err == hipSuccess ? ms : -1.f;
+    };
+    const bool lds_ok = !fused && W.lds_allowed;
+    // with the LDS-tiled kernels forced, the first sweep runs them too
+    const bool forced = lds_ok && ctx->stencil == FLOF_STENCIL_LDS;
+    flof_ctx::TunedShape best{ctx->tm_blocks_per_cu, forced ? ctx->lds_blocks_per_cu : 0};
+    float best_ms = 1e30f;
+    for (int bpc : TM_CANDIDATES) {
+        W.set_tm(ctx, D, bpc);
+        W.lds = forced;
+        if (W.nb_tm > W.cap || W.nb_lds > W.cap) continue;
+        const float ms = time_it();
+        if (ms < 0.f) break;
         if (ms < best_ms) {
             best_ms = ms;
-            best = bpc;
+            best = {bpc, forced ? ctx->lds_blocks_per_cu : 0};
+        }
+    }
+    if (lds_ok && err == hipSuccess) {
+        const int bpc = best.bpc;
+        int last_nseg = -1;
+        for (int lb : LDS_CANDIDATES) {
+            W.set_tm(ctx, D, bpc, lb);
+            W.lds = true;
+            if (W.nb_tm > W.cap || W.nb_lds > W.cap || W.tl.nseg == last_nseg) continue;
+            last_nseg = W.tl.nseg;
+            const float ms = time_it();
+            if (ms < 0.f) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = {bpc, lb};
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -2114,10 +2151,13 @@ static void level_setup(flof_ctx *ctx, LevelCg &L, const void *i0, const void *i
     void **pb = L.pb;
     direction_buffers(W, pstore, n, pb);
     if (ctx->use_tmarch) {
-        int bpc = ctx->tm_blocks_per_cu;
-        if (ctx->autotune && n >= (1 << 16))
-            bpc = tuned_bpc(ctx, D, W, i1, G, dxf, pstore, pb, fused);
-        W.set_tm(ctx, D, bpc);
+        if (ctx->autotune && n >= (1 << 16)) {
+            const flof_ctx::TunedShape ts = tuned_shape(ctx, D, W, i1, G, dxf, pstore, pb, fused);
+            W.set_tm(ctx, D, ts.bpc, ts.lds_bpc);
+            W.lds = ts.lds_bpc > 0 && W.lds_allowed;
+        } else {
+            W.set_tm(ctx, D, ctx->tm_blocks_per_cu);
+        }
         if (fused) W.lds = false;
     }
     // with fewer than 3 cells in a dimension every row is an identity row and
@@ -2218,15 +2258,12 @@ static SolveOut level_finish(flof_ctx *ctx, LevelCg &L, float4 *out, bool accumu
 
 // Solves one level.  vin: incoming flow for the rhs (may be null = zero).
 // The result x/h is written to (accumulate ? += : =) out.
-SolveOut solve_level(flof_ctx *ctx, const void *i0, const void *i1, const float4 *vin,
-                     float *rhsT, const Dims4 &D, float wSmooth, float wEnergy,
-                     float cgAccuracy, float dt, int maxIter, const float4 *warm,
-                     const float4 *warm_base, float4 *out, bool accumulate)
+// The PCG iterations of one level set up by level_setup (fused = false),
+// until it converges or reaches maxIter; the final state goes to L.hs.
+static void level_iterate(flof_ctx *ctx, LevelCg &L, int maxIter)
 {
-    LevelCg L(ctx, D);
-    level_setup(ctx, L, i0, i1, vin, rhsT, wSmooth, wEnergy, cgAccuracy, dt, maxIter, warm,
-                warm_base, false);
     LevelWork &W = L.W;
+    const void *i1 = L.i1;
     CgState *st = W.st.as<CgState>();
     const int64_t n = L.n;
     const float dxf = L.dxf;
@@ -2323,6 +2360,17 @@ SolveOut solve_level(flof_ctx *ctx, const void *i0, const void *i1, const float4
     FLOF_HIP(loop_err);
     check_launch("k_cg_*");
     L.hs = hs;
+}
+
+SolveOut solve_level(flof_ctx *ctx, const void *i0, const void *i1, const float4 *vin,
+                     float *rhsT, const Dims4 &D, float wSmooth, float wEnergy,
+                     float cgAccuracy, float dt, int maxIter, const float4 *warm,
+                     const float4 *warm_base, float4 *out, bool accumulate)
+{
+    LevelCg L(ctx, D);
+    level_setup(ctx, L, i0, i1, vin, rhsT, wSmooth, wEnergy, cgAccuracy, dt, maxIter, warm,
+                warm_base, false);
+    level_iterate(ctx, L, maxIter);
     return level_finish(ctx, L, out, accumulate);
 }
 
@@ -2782,11 +2830,15 @@ static float multiscale(flof_ctx *ctx, float4 *vel, const void *i0, const void *
 // of that index together, their PCG iterations in shared launches
 // (fused_iterations).  Per problem the operations and their order are those
 // of multiscale(), so every flow and record is the one a single call gives.
+// cells from which a level of a fused batch may run on its own
+static const int64_t SOLO_MIN_CELLS = (int64_t)1 << 22;
+
 static void multiscale_fused(flof_ctx *ctx, int np, float4 *const *vel_top,
                              const void *const *i0_top, const void *const *i1_top,
                              const Dims4 *D_top, const MsParams &P, float dt,
                              flof_solve_info *const *info, float *errs)
 {
+    FLOF_REQUIRE(ctx->use_tmarch, FLOF_ERR_ARG, "fused batch: needs the t-marching kernels");
     const bool bf = ctx->img_bf16;
     const size_t ib = bf ? 2 : 4;
     const int resetBnd = 0;  // the band of resetBndWidth is not part of this form
@@ -2879,12 +2931,23 @@ static void multiscale_fused(flof_ctx *ctx, int np, float4 *const *vel_top,
                                     ctx->stream));
             prewarp(q.vel[l], w, D);
             if (info[j] && l < FLOF_MAX_LEVELS) info[j]->err_level[l] = lsdiff(w, q.i1[l], D);
+            // A level large enough to fill the chip on its own whose single
+            // form runs the LDS-tiled stencil kernels is solved on its own
+            // (those kernels run one problem per launch); the others share
+            // the launches of the fused iterations.  Either way the result
+            // is the problem's own solve, bit for bit.
             Ls.emplace_back(new LevelCg(ctx, D));
+            const bool may_solo = n >= SOLO_MIN_CELLS;
             level_setup(ctx, *Ls.back(), w, q.i1[l], nullptr, nullptr, P.wSmooth, P.wEnergy,
-                        P.cgAccuracy, dt, P.cgMaxIter, nullptr, nullptr, true);
+                        P.cgAccuracy, dt, P.cgMaxIter, nullptr, nullptr, !may_solo);
         }
         std::vector<LevelCg *> raw;
-        for (auto &L : Ls) raw.push_back(L.get());
+        for (auto &L : Ls) {
+            if (L->W.lds)
+                level_iterate(ctx, *L, P.cgMaxIter);
+            else
+                raw.push_back(L.get());
+        }
         fused_iterations(ctx, raw);
         for (size_t a = 0; a < act.size(); ++a) {
             const int j = act[a];

@@ -42,12 +42,17 @@ def test_batch_equals_single_solves():
         assert infos[n].summary()["levels"] == inf["levels"]
 
 
-def test_autotune_keeps_results():
+@pytest.mark.parametrize("shape", [(24, 24, 24, 20), (70, 16, 16, 24)])
+def test_autotune_keeps_results(shape):
     """The launch-shape autotuner changes timing only: a solve with it on is
-    identical to one with it off, and it records a candidate per shape."""
+    identical to one with it off, and it records a valid choice for the
+    finest level's shape (t-marching workgroups per CU; LDS-tiled or
+    t-marching stencil kernels and the tiles' t-segment length).  The
+    second shape has a ragged second 64-wide x tile."""
+    import ctypes as C
     from ofblend_amd import _lib
-    X, Y, Z, T = 24, 24, 24, 20
-    print("autotune", (X, Y, Z, T), flush=True)
+    X, Y, Z, T = shape
+    print("autotune", shape, flush=True)
     s = M.Solver(gridSize=M.vec3(X, Y, Z), fourthDim=T)
     g0, g1, v = s.create(M.Grid4Real), s.create(M.Grid4Real), s.create(M.Grid4Vec4)
     g0.from_numpy(O.drop_sdf(X, Y, Z, T, seed=31).astype(np.float32))
@@ -62,6 +67,17 @@ def test_autotune_keeps_results():
         M.opticalFlowMultiscale4d(vel=v, i0=g0, i1=g1, **P)
         assert np.array_equal(v.to_numpy(), ref)
         assert L.flof_ctx_tuned_blocks(ctx, X, Y, Z, T) in (2, 4, 8, 16)
+        tb, lb, sl = C.c_int(), C.c_int(), C.c_int()
+        assert L.flof_ctx_tuned_shape(ctx, X, Y, Z, T, C.byref(tb), C.byref(lb),
+                                      C.byref(sl)) == 0
+        print("tuned: t-march", tb.value, "LDS work items per CU", lb.value,
+              "t segment", sl.value, flush=True)
+        assert tb.value in (2, 4, 8, 16) and lb.value in (0, 1, 2, 4, 8)
+        assert (sl.value == 0) == (lb.value == 0) and 0 <= sl.value <= T
+        # a second solve reuses the choice and gives the same result again
+        v.clear()
+        M.opticalFlowMultiscale4d(vel=v, i0=g0, i1=g1, **P)
+        assert np.array_equal(v.to_numpy(), ref)
     finally:
         L.flof_ctx_set_autotune(ctx, 0)
 
@@ -121,6 +137,30 @@ def test_fused_batch_equals_single_solves():
         ref.append((e, v.to_numpy(), M.getLastSolveInfo().summary()))
         v.clear()
     print("fused batch vs single solves", len(jobs), flush=True)
+    errs, infos = M.opticalFlowMultiscale4dFused([j[3] for j in jobs], [j[1] for j in jobs],
+                                                 [j[2] for j in jobs], **prm)
+    for n, j in enumerate(jobs):
+        e, vr, inf = ref[n]
+        assert errs[n] == e
+        assert np.array_equal(j[3].to_numpy(), vr), n
+        assert infos[n].summary()["levels"] == inf["levels"]
+
+
+def test_fused_batch_with_a_level_solved_on_its_own():
+    """A finest level of 64^3 x 32 (8.4M cells, full 64-wide tiles, 16-step
+    t segments) runs the LDS-tiled stencil kernels on its own while the
+    other problems' levels share the fused launches: every result is still
+    the single solve's, bit for bit."""
+    shapes = [(64, 64, 64, 32), (24, 20, 18, 16), (40, 40, 40, 24)]
+    jobs = _jobs(shapes, 80)
+    prm = dict(P, minGridSize=16)
+    ref = []
+    for s, g0, g1, v, a, b in jobs:
+        v.clear()
+        e = M.opticalFlowMultiscale4d(vel=v, i0=g0, i1=g1, **prm)
+        ref.append((e, v.to_numpy(), M.getLastSolveInfo().summary()))
+        v.clear()
+    print("fused batch with a solo level", flush=True)
     errs, infos = M.opticalFlowMultiscale4dFused([j[3] for j in jobs], [j[1] for j in jobs],
                                                  [j[2] for j in jobs], **prm)
     for n, j in enumerate(jobs):

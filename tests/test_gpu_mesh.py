@@ -75,9 +75,10 @@ def test_mesh_bit_exact(name, tmp_path):
 
 
 def test_mesh_of_warped_frame():
-    """A frame of the output pipeline: the drop warped by a random smooth
-    deformation (loadAdvectTimeSlice_OptRun), then meshed."""
-    N, T, F = 12, 8, 2
+    """A frame of the output pipeline: the drop warped by a random
+    deformation (loadAdvectTimeSlice_OptRun, which writes only cells 10 or
+    more from the border), then meshed."""
+    N, T, F = 24, 8, 2
     rng = np.random.default_rng(9)
     frames = O.drop_sdf(N, N, N, T, seed=4, scale=F)
     phi_np = O.place_frames((T, N * F, N * F, N * F), frames)

@@ -558,3 +558,35 @@ def test_stencil_kernels_degenerate_shapes():
                 assert np.array_equal(r[0], res[0][0])
     finally:
         L.flof_ctx_set_stencil(ctx, FLOF_STENCIL_AUTO)
+
+
+@pytest.mark.parametrize("where,bad", [("i1", np.nan), ("i0", np.nan), ("i1", np.inf),
+                                       ("i0", np.inf)])
+def test_non_finite_image_gives_the_references_zero_flow(where, bad):
+    """A NaN or Inf in an input image makes the CG's first sigma NaN: the
+    reference stops before the first iteration and keeps a zero increment
+    (optflow4d.cpp:297-301, 508-514).  The GPU level solve returns the
+    oracle's flow bit for bit, status 2 and 0 iterations.  (A NaN that
+    appeared only later would end the GPU's CG at once with the zero
+    increment, where the reference iterates to its cap first and then
+    resets; the flow is the same.)"""
+    N, T = 12, 10
+    i0, i1 = _drops(N, T)
+    imgs = {"i0": i0.copy(), "i1": i1.copy()}
+    imgs[where][T // 2, N // 2, N // 2, N // 2 + 1] = bad
+    say("non-finite", where, bad)
+    s = solver(N, N, N, T)
+    vin = (RNG.standard_normal((T, N, N, N, 4)) * 0.5).astype(np.float32)
+    vel = vec4(s, vin)
+    prm = dict(wSmooth=0.1, wEnergy=1e-4, cgAccuracy=1e-5, cgMaxIter=60)
+    M.opticalFlow4d(vel, real(s, imgs["i0"]), real(s, imgs["i1"]), **prm)
+    lv = M.getLastSolveInfo().summary()["levels"][0]
+    p = default_params(**prm)
+    with O.sum_mode(1):
+        ref, it, res, st = O.optical_flow4d(vin, imgs["i0"], imgs["i1"], p)
+    say("gpu iters", lv["iters"], "status", lv["status"], "oracle", it, st)
+    got = vel.to_numpy()
+    assert np.array_equal(got, ref, equal_nan=True)
+    assert np.isfinite(got).all()
+    assert lv["status"] == [st] and st == 2
+    assert lv["iters"] == [it] == [0]
