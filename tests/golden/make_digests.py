@@ -58,7 +58,8 @@ def dist(a, b):
 
 
 def warp_frames(N, flow, dtype):
-    """The first, middle and last frame of the C2-style frame warp."""
+    """The first, middle and last frame of the frame warp: T frames at twice
+    the resolution, placed 1:1 onto t, warped by the flow."""
     T = N
     frames = O.drop_sdf(N, N, N, T, seed=1, scale=WARP_SCALE, dtype=dtype)
     F = WARP_SCALE * N

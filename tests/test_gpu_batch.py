@@ -170,6 +170,34 @@ def test_fused_batch_with_a_level_solved_on_its_own():
         assert infos[n].summary()["levels"] == inf["levels"]
 
 
+def test_fused_batch_at_batch_workload_sizes():
+    """Two of the 16-solve batch's problems at full size (80^3 x 64 and
+    48^3 x 32, forward and backward, 2 spheres): the fused batch equals the
+    single solves bit for bit and every level reaches the residual target.
+    (The alignment error need not fall: with two spheres far apart the
+    linearised flow can raise calcLsDiff above the unwarped pair's.)"""
+    shapes = [(80, 80, 80, 64), (48, 48, 48, 32)]
+    jobs = _jobs(shapes, 100, nspheres=2)
+    jobs += [(s, g1, g0, s.create(M.Grid4Vec4), b, a) for (s, g0, g1, v, a, b) in jobs]
+    prm = dict(P, minGridSize=16, cgMaxIter=20000)
+    ref = []
+    for s, g0, g1, v, a, b in jobs:
+        v.clear()
+        e = M.opticalFlowMultiscale4d(vel=v, i0=g0, i1=g1, **prm)
+        ref.append((e, v.to_numpy(), M.getLastSolveInfo().summary()))
+        v.clear()
+    print("fused batch at batch-workload sizes", flush=True)
+    errs, infos = M.opticalFlowMultiscale4dFused([j[3] for j in jobs], [j[1] for j in jobs],
+                                                 [j[2] for j in jobs], **prm)
+    for n, j in enumerate(jobs):
+        e, vr, inf = ref[n]
+        got = infos[n].summary()
+        assert errs[n] == e
+        assert np.array_equal(j[3].to_numpy(), vr), n
+        assert got["levels"] == inf["levels"]
+        assert all(r <= 1e-5 for l in got["levels"] for r in l["residual"])
+
+
 def test_fused_batch_rejects_other_parameters():
     s = M.Solver(gridSize=M.vec3(8, 8, 8), fourthDim=8)
     g0, g1, v = s.create(M.Grid4Real), s.create(M.Grid4Real), s.create(M.Grid4Vec4)

@@ -94,7 +94,7 @@ def test_mixed_images_level_solve_bit_exact():
 
 @pytest.mark.parametrize("multiStep,blur", [(1, 0.0), (3, 2.0)])
 def test_mixed_images_multiscale_bit_exact_and_error(multiStep, blur):
-    """24^3 x 20, the C3 generator (2 spheres, radii 0.08N-0.18N): the
+    """24^3 x 20, the mixed-precision generator (2 spheres, radii 0.08N-0.18N): the
     mixed-precision solve equals the oracle that rounds at the same points
     (images stored as bfloat16 at the start, at each pyramid level and after
     every warp; gradients rounded to binary16); its distance to the FP64

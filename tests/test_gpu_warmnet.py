@@ -157,3 +157,41 @@ def test_capture_and_net_level_start_bit_exact():
         "kept", sn["levels"][0]["warm_used"])
     assert sn["levels"][0]["iters"][0] == it_n
     assert np.array_equal(vn.to_numpy(), vp + ref_n)
+
+
+def test_trained_network_saves_iterations_on_heldout_pairs():
+    """The shipped network (trained on the finest-level increments of 2000
+    plain 64^4 solves, python -m ofblend_amd.train_warmnet) on 64^4 drop
+    pairs outside its training seeds: with every level's CG started from the
+    network's guess of that level's increment, each solve reaches the same
+    residual target, its alignment error stays the cold solve's within
+    0.1%, and the finest levels of the three pairs need fewer iterations in
+    total than the cold solves -- by little: 4873 against 4961 (1.8%) when
+    this was written; the solves are deterministic, so the counts repeat."""
+    import os
+    if not os.path.exists(WS.NET64_WEIGHTS):
+        pytest.skip("no trained weights")
+    N = 64
+    s = M.Solver(gridSize=M.vec3(N, N, N), fourthDim=N)
+    net = WS.WarmNet(seed=1)
+    net.load(WS.NET64_WEIGHTS)
+    prm = dict(WS.PLAIN, minGridSize=16, cgMaxIter=20000)
+    cold_it, net_it = 0, 0
+    for k in range(3):
+        i0, i1 = WS.drop_pair(s, 10 ** 6 + 2 * k, nspheres=1 + k % 3, rmin=0.08, rmax=0.18)
+        v0 = s.create(M.Grid4Vec4)
+        e0 = M.opticalFlowMultiscale4d(vel=v0, i0=i0, i1=i1, **prm)
+        a0 = [l["iters"][0] for l in M.getLastSolveInfo().summary()["levels"]]
+        v1 = s.create(M.Grid4Vec4)
+        e1, inf = net.solve(v1, i0, i1, levels=-1, **prm)
+        lv = inf.summary()["levels"]
+        a1 = [l["iters"][0] for l in lv]
+        say("held-out pair %d: iterations cold %s net %s kept %s err %.4f / %.4f"
+            % (k, a0, a1, [l["warm_used"] for l in lv], e0, e1))
+        assert all(r <= 1e-5 for l in lv for r in l["residual"])
+        assert abs(e1 - e0) <= 1e-3 * abs(e0)
+        cold_it += a0[0]
+        net_it += a1[0]
+    say("finest-level iterations, 3 held-out pairs: cold %d net %d (ratio %.3f)"
+        % (cold_it, net_it, cold_it / net_it))
+    assert net_it < cold_it
