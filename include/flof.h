@@ -411,7 +411,10 @@ int flof_optical_flow_multiscale4d_batch(flof_ctx *ctx, int n,
  * solved together, every PCG iteration as one launch per kernel kind that
  * spans all of them (a work-item table maps each workgroup to its problem
  * and its solver state; converged problems return at once), chunks of
- * iterations captured as HIP graphs.  Runs on ctx's stream; each result is
+ * iterations captured as HIP graphs.  A level of 2^22 cells or more whose
+ * single solve would run the LDS-tiled stencil kernels iterates on its own
+ * instead (those kernels run one problem per launch; the level fills the
+ * chip by itself).  Runs on ctx's stream; each result is
  * bit-identical to a flof_optical_flow_multiscale4d call.  The finest levels
  * of all problems are resident at once (96 B of PCG workspace per cell). */
 int flof_optical_flow_multiscale4d_fused(flof_ctx *ctx, int n, flof_grid4d *const *vel,

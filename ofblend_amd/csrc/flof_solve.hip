@@ -1294,12 +1294,10 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
                 if (!own[q]) continue;
                 const int k = z0 + q;
                 float4 qc, pr;
-                // Branch-free: every lane evaluates the interior row.  On an
-                // outer-layer cell it runs with a neighbour sum of +0, g = 0
-                // and a diagonal of 1, which makes it the identity row
-                // ((((0 + 1 p_d) + 0 p_m ...), 1 * p_d = p_d) bit for bit and
-                // the preconditioner 1 / (0 + 1) = 1; the neighbour values it
-                // reads there lie in the tile's LDS slots and are discarded.
+                // Branch-free: every lane evaluates the interior row; on an
+                // outer-layer cell the result is replaced by the identity
+                // row's (the neighbour values read there lie in the tile's
+                // LDS slots and are discarded).
                 const bool inner = sp_xy && k >= 1 && k < D.Z - 1 && t >= 1 && t < D.T - 1;
                 {
                     float4 v = zero4;
@@ -1333,11 +1331,6 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
                         gc.z = (izq - izn) * dxf;
                     }
                     gc.w = (ip[q] - im[q]) * dxf;
-                    v = make_float4(inner ? v.x : 0.f, inner ? v.y : 0.f, inner ? v.z : 0.f,
-                                    inner ? v.w : 0.f);
-                    gc = make_float4(inner ? gc.x : 0.f, inner ? gc.y : 0.f,
-                                     inner ? gc.z : 0.f, inner ? gc.w : 0.f);
-                    const float dg = inner ? G.diag : 1.f;
                     const float gg[4] = {gc.x, gc.y, gc.z, gc.w};
                     const float pv[4] = {pc[q].x, pc[q].y, pc[q].z, pc[q].w};
                     float o[4] = {v.x, v.y, v.z, v.w};
@@ -1346,12 +1339,19 @@ k_cg_stencil_lds(const typename PS::T *__restrict__ p, const float *__restrict__
 #pragma unroll
                         for (int m = 0; m < 4; ++m) {
                             float b = gg[d] * gg[m];
-                            if (d == m) b = b + dg;
+                            if (d == m) b = b + G.diag;
                             o[d] = o[d] + b * pv[m];
                         }
                     }
-                    qc = make_float4(o[0], o[1], o[2], o[3]);
-                    if (UPDATE) pr = prec_f32(gc, dg, G.fast_rcp);
+                    // an outer-layer cell's row is the identity: q = p and
+                    // the preconditioner 1 (what the interior row gives with
+                    // a neighbour sum of +0, g = 0 and a diagonal of 1, for
+                    // the +0 that p always holds there)
+                    qc = inner ? make_float4(o[0], o[1], o[2], o[3]) : pc[q];
+                    if (UPDATE) {
+                        pr = prec_f32(gc, G.diag, G.fast_rcp);
+                        if (!inner) pr = make_float4(1.f, 1.f, 1.f, 1.f);
+                    }
                 }
                 if (!UPDATE) {
                     dd_add4(acc, pc[q], qc);
