@@ -195,7 +195,8 @@ typedef struct flof_mesh flof_mesh;
  * nodes and triangles are numbered as the reference numbers them.  The
  * mesh's normals are those its file holds (Mesh::computeVertexNormals,
  * source/mesh.cpp:460-479); the mesher's own gradient normals, which
- * writeBobjFile replaces, are not formed. */
+ * writeBobjFile replaces, are not formed.  Grids of 2^31/5 points or more
+ * are rejected (the counts are 32-bit). */
 int flof_create_mesh(flof_ctx *ctx, const float *phi_dev, int X, int Y, int Z,
                      flof_mesh **out);
 int flof_mesh_size(const flof_mesh *m, int64_t *nodes, int64_t *tris);

@@ -400,7 +400,9 @@ int flof_create_mesh(flof_ctx *ctx, const float *phi, int X, int Y, int Z, flof_
     *out = nullptr;
     FLOF_REQUIRE(X >= 2 && Y >= 2 && Z >= 2, FLOF_ERR_DIM,
                  "createMesh: only 3D grids (at least 2 cells per dimension)");
-    FLOF_REQUIRE((int64_t)X * Y * Z < (1ll << 31) / 3, FLOF_ERR_DIM, "createMesh: grid too large");
+    // vertex (3 per grid point) and triangle (5 per cube) counts and their
+    // scans are 32-bit
+    FLOF_REQUIRE((int64_t)X * Y * Z < (1ll << 31) / 5, FLOF_ERR_DIM, "createMesh: grid too large");
     MeshGeo g{X, Y, Z, (int64_t)(X - 1) * (Y - 1) * (Z - 1)};
     const int64_t ng = (int64_t)X * Y * Z;
     Tmp cases(ctx, (size_t)g.nc * 2), vb(ctx, (size_t)g.nc * 4), tb(ctx, (size_t)g.nc * 4),

@@ -47,6 +47,36 @@ inline V3 normalize3(V3 v)
     return V3{0.f, 0.f, 0.f};
 }
 
+// computeVertexNormals: per triangle, in triangle order, corner c gets
+// cross(p0 - p1, p1 - p2) / (|edge before c|^2 |edge after c|^2), then every
+// normal is normalised
+std::vector<V3> vertex_normals(const std::vector<V3> &pos, const int32_t *tris, int64_t nt)
+{
+    std::vector<V3> nrm(pos.size(), V3{0.f, 0.f, 0.f});
+    for (int64_t t = 0; t < nt; ++t) {
+        const int32_t *c = &tris[3 * t];
+        const V3 p0 = pos[c[0]], p1 = pos[c[1]], p2 = pos[c[2]];
+        const V3 n0{p0.x - p1.x, p0.y - p1.y, p0.z - p1.z};
+        const V3 n1{p1.x - p2.x, p1.y - p2.y, p1.z - p2.z};
+        const V3 n2{p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+        const float l0 = n0.x * n0.x + n0.y * n0.y + n0.z * n0.z;
+        const float l1 = n1.x * n1.x + n1.y * n1.y + n1.z * n1.z;
+        const float l2 = n2.x * n2.x + n2.y * n2.y + n2.z * n2.z;
+        const V3 nm{(n0.y * n1.z) - (n0.z * n1.y), (n0.z * n1.x) - (n0.x * n1.z),
+                    (n0.x * n1.y) - (n0.y * n1.x)};
+        const double f[3] = {1.0 / (double)(l0 * l2), 1.0 / (double)(l0 * l1),
+                             1.0 / (double)(l1 * l2)};
+        for (int q = 0; q < 3; ++q) {
+            V3 &a = nrm[c[q]];
+            a.x = a.x + (float)((double)nm.x * f[q]);
+            a.y = a.y + (float)((double)nm.y * f[q]);
+            a.z = a.z + (float)((double)nm.z * f[q]);
+        }
+    }
+    for (auto &n : nrm) n = normalize3(n);
+    return nrm;
+}
+
 }  // namespace
 
 extern "C" int orc_create_mesh(const float *phi, int X, int Y, int Z, float *pos_out,
@@ -110,37 +140,30 @@ extern "C" int orc_create_mesh(const float *phi, int X, int Y, int Z, float *pos
     *nv_out = nv;
     *nt_out = nt;
     if (nv > maxv || nt > maxt) return 1;
-    // computeVertexNormals
-    std::vector<V3> nrm(pos.size(), V3{0.f, 0.f, 0.f});
-    for (int64_t t = 0; t < nt; ++t) {
-        const int32_t *c = &tris[3 * t];
-        const V3 p0 = pos[c[0]], p1 = pos[c[1]], p2 = pos[c[2]];
-        const V3 n0{p0.x - p1.x, p0.y - p1.y, p0.z - p1.z};
-        const V3 n1{p1.x - p2.x, p1.y - p2.y, p1.z - p2.z};
-        const V3 n2{p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
-        const float l0 = n0.x * n0.x + n0.y * n0.y + n0.z * n0.z;
-        const float l1 = n1.x * n1.x + n1.y * n1.y + n1.z * n1.z;
-        const float l2 = n2.x * n2.x + n2.y * n2.y + n2.z * n2.z;
-        const V3 nm{(n0.y * n1.z) - (n0.z * n1.y), (n0.z * n1.x) - (n0.x * n1.z),
-                    (n0.x * n1.y) - (n0.y * n1.x)};
-        const double f[3] = {1.0 / (double)(l0 * l2), 1.0 / (double)(l0 * l1),
-                             1.0 / (double)(l1 * l2)};
-        for (int q = 0; q < 3; ++q) {
-            V3 &a = nrm[c[q]];
-            a.x = a.x + (float)((double)nm.x * f[q]);
-            a.y = a.y + (float)((double)nm.y * f[q]);
-            a.z = a.z + (float)((double)nm.z * f[q]);
-        }
-    }
+    const std::vector<V3> nrm = vertex_normals(pos, tris.data(), nt);
     for (int64_t v = 0; v < nv; ++v) {
-        const V3 n = normalize3(nrm[v]);
         pos_out[3 * v + 0] = pos[v].x;
         pos_out[3 * v + 1] = pos[v].y;
         pos_out[3 * v + 2] = pos[v].z;
-        nrm_out[3 * v + 0] = n.x;
-        nrm_out[3 * v + 1] = n.y;
-        nrm_out[3 * v + 2] = n.z;
+        nrm_out[3 * v + 0] = nrm[v].x;
+        nrm_out[3 * v + 1] = nrm[v].y;
+        nrm_out[3 * v + 2] = nrm[v].z;
     }
     std::memcpy(tri_out, tris.data(), tris.size() * sizeof(int32_t));
     return 0;
+}
+
+// Mesh::computeVertexNormals on given nodes and triangles (the routine the
+// mesher uses, exposed so that its weighting can be checked on its own).
+extern "C" void orc_vertex_normals(const float *pos, int64_t nv, const int32_t *tris,
+                                   int64_t nt, float *nrm_out)
+{
+    std::vector<V3> p(nv);
+    for (int64_t v = 0; v < nv; ++v) p[v] = V3{pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]};
+    const std::vector<V3> n = vertex_normals(p, tris, nt);
+    for (int64_t v = 0; v < nv; ++v) {
+        nrm_out[3 * v + 0] = n[v].x;
+        nrm_out[3 * v + 1] = n[v].y;
+        nrm_out[3 * v + 2] = n[v].z;
+    }
 }

@@ -85,6 +85,8 @@ def _declare(L):
     L.orc_create_mesh.argtypes = [_P, _I, _I, _I, _P, _P, _P, _I64, _I64,
                                   C.POINTER(_I64), C.POINTER(_I64)]
     L.orc_create_mesh.restype = _I
+    L.orc_vertex_normals.argtypes = [_P, _I64, _P, _I64, _P]
+    L.orc_vertex_normals.restype = None
     for name in ("orc_round_bf16", "orc_round_f16"):
         getattr(L, name).argtypes = [C.c_float]
         getattr(L, name).restype = C.c_float
@@ -427,6 +429,17 @@ def extrap4d_ls_simple(phi, distance=4, inside=False):
     g = np.array(phi, copy=True, order="C")
     _fn("orc_extrap4d_ls_simple", g)(_ptr(g), *_dims(g), distance, int(inside))
     return g
+
+
+def vertex_normals(pos, tris):
+    """Mesh::computeVertexNormals (mesh.cpp:460-479) of nodes pos (V, 3) and
+    triangles tris (F, 3): the normals (V, 3), float32."""
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    out = np.empty_like(pos)
+    lib().orc_vertex_normals(pos.ctypes.data, len(pos), tris.ctypes.data, len(tris),
+                              out.ctypes.data)
+    return out
 
 
 def create_mesh(phi):

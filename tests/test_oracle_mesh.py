@@ -133,3 +133,42 @@ def test_invalid_cubes_are_skipped():
     p_full, _, t_full = O.create_mesh(phi)
     assert 0 < len(t) < len(t_full)
     assert p[:, 0].max() <= 9.0 + 1e-6  # no vertex beyond the last valid cube
+
+
+def test_vertex_normal_weights_are_exact_on_a_sphere():
+    """Mesh::computeVertexNormals weights each triangle's normal at a corner by
+    sin(angle) / (|e1| |e2|) of the two edges there (mesh.cpp:460-479), the
+    weighting for which Max (1999, "Weights for computing vertex normals from
+    facet normals") shows the vertex normal is exact when the vertex and its
+    neighbours lie on one sphere.  An irregular closed fan on a sphere: the
+    oracle's normal is the radial direction to float precision, while the
+    area- and angle-weighted normals of the same fan are measurably off (so
+    the check tells the weightings apart)."""
+    rng = np.random.default_rng(12)
+    c = np.array([0.3, -0.2, 0.1])
+    r = 1.7
+    k = 7
+    az = np.sort(rng.uniform(0, 2 * np.pi, k))
+    polar = rng.uniform(0.15, 0.6, k)
+    # vertex at the sphere's top, neighbours around it on the sphere
+    v = c + np.array([0, 0, r])
+    nb = c + r * np.stack([np.sin(polar) * np.cos(az), np.sin(polar) * np.sin(az),
+                           np.cos(polar)], 1)
+    pos = np.vstack([v, nb]).astype(np.float32)
+    tris = np.array([[0, 1 + i, 1 + (i + 1) % k] for i in range(k)], np.int32)
+    n = O.vertex_normals(pos, tris)[0].astype(np.float64)
+    radial = (pos[0] - c) / np.linalg.norm(pos[0] - c)
+    assert np.dot(n, radial) > 0  # counter-clockwise fan seen from outside
+    assert np.abs(n - radial).max() < 2e-6
+    # the other common weightings miss the radial direction on this fan
+    p = pos.astype(np.float64)
+    area, angle = np.zeros(3), np.zeros(3)
+    for a, b, d in tris:
+        e1, e2 = p[b] - p[a], p[d] - p[a]
+        cr = np.cross(e1, e2)
+        area += cr
+        ang = np.arccos(np.dot(e1, e2) / np.linalg.norm(e1) / np.linalg.norm(e2))
+        angle += ang * cr / np.linalg.norm(cr)
+    for w in (area, angle):
+        w = w / np.linalg.norm(w)
+        assert np.abs(w - radial).max() > 1e-3
