@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--cpu-calibration", action="store_true",
                     help="CPU only: the oracle's full 32^4 solve timed against the "
                          "bounded-sample estimate the CPU baseline uses; prints JSON")
+    ap.add_argument("--mesh-workload", action="store_true",
+                    help="createMesh of warped 128^3 frames (a short solve first); "
+                         "prints its own JSON line")
     ap.add_argument("--ops-workload", action="store_true",
                     help="the path's other operations at 64^4 (grid ops, advect4d, "
                          "interpolateGrid4d, gaussianBlur4d, calcLsDiff4d, "
@@ -137,6 +140,11 @@ def main():
         M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, cgMaxIter=50, **PARAMS)
         M.sync()
         print(json.dumps(frame_warp(M, L, ctx, s, vel, FLOF_KT_WARP)), flush=True)
+        return
+    if args.mesh_workload:
+        M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, cgMaxIter=50, **PARAMS)
+        M.sync()
+        print(json.dumps(mesh_workload(M, L, ctx, s, vel)), flush=True)
         return
     if args.ops_workload:
         M.opticalFlowMultiscale4d(vel=vel, i0=i0, i1=i1, **PARAMS)
@@ -529,10 +537,10 @@ def committed_traffic(kernel):
     return None
 
 
-def frame_warp(M, L, ctx, s, vel, kind):
+def frame_warp(M, L, ctx, s, vel, kind, keep=None):
     """The frame warp workload: 64 analytic 3D frames at 128^3 placed into a
     128^3 x 64 level set and warped by the solved 64^4 deformation (alpha 1,
-    defoFactor (2,2,2,1))."""
+    defoFactor (2,2,2,1)).  keep (a dict) receives the warped frames."""
     F = 2
     fr = M.DeviceArray((N, F * N, F * N, F * N))
     M.dropFrames3d(fr, N, N, N, N, F, seed=1)
@@ -564,11 +572,44 @@ def frame_warp(M, L, ctx, s, vel, kind):
     # level set (N time slices of M3 cells) and the deformation (N slices of
     # Md Vec4 cells) once each -- consecutive frames share their slices
     byts = N * 4 * M3 + N * 4 * M3 + N * 16 * Md
+    if keep is not None:
+        keep["frames"] = dst
     return {"frames": N, "frame": [F * N] * 3, "ms_per_64_frames": 1000 * wall,
             "frames_per_s": N / wall, "kernel_ms": 1000 * kern_s,
             "bytes_per_launch": byts,
             "GBps_algorithmic": byts / kern_s / 1e9, "frac": byts / kern_s / HBM_PEAK,
             "place_ms": 1000 * t_place}
+
+
+def mesh_workload(M, L, ctx, s, vel, reps=5):
+    """createMesh of warped output frames (the warp workload's frames 0, 31
+    and 63 at 128^3): wall time per mesh (the mesher synchronises: count
+    pass, two scans read back to size the arrays, vertices, triangles,
+    normals), nodes and triangles."""
+    from ofblend_amd.abi import FLOF_KT_WARP
+    keep = {}
+    frame_warp(M, L, ctx, s, vel, FLOF_KT_WARP, keep)
+    dst = keep["frames"]
+    F = 2
+    M3 = (F * N) ** 3
+    out = {"frame": [F * N] * 3, "frames": {}}
+    for t in (0, N // 2 - 1, N - 1):
+        ptr = C.c_void_p(dst.ptr.value + t * M3 * 4)
+        h = C.c_void_p()
+        _lib_check(L.flof_create_mesh(ctx, ptr, F * N, F * N, F * N, C.byref(h)), ctx)
+        L.flof_mesh_destroy(h)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _lib_check(L.flof_create_mesh(ctx, ptr, F * N, F * N, F * N, C.byref(h)), ctx)
+            if _ != reps - 1:
+                L.flof_mesh_destroy(h)
+        wall = (time.perf_counter() - t0) / reps
+        nv, nt = C.c_int64(), C.c_int64()
+        L.flof_mesh_size(h, C.byref(nv), C.byref(nt))
+        L.flof_mesh_destroy(h)
+        out["frames"][str(t)] = {"ms": 1000 * wall, "nodes": nv.value, "triangles": nt.value,
+                                 "cells_per_s": M3 / wall}
+    return out
 
 
 def ops_workload(M, s, i0, i1, vel, reps=10):
