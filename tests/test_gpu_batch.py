@@ -198,6 +198,19 @@ def test_fused_batch_at_batch_workload_sizes():
         assert all(r <= 1e-5 for l in got["levels"] for r in l["residual"])
 
 
+def test_fused_batch_of_one_problem():
+    """A batch of one is the single solve."""
+    jobs = _jobs([(20, 18, 16, 12)], 120, nspheres=2)
+    s, g0, g1, v, a, b = jobs[0]
+    prm = dict(P, minGridSize=8)
+    e = M.opticalFlowMultiscale4d(vel=v, i0=g0, i1=g1, **prm)
+    ref, inf = v.to_numpy(), M.getLastSolveInfo().summary()
+    v.clear()
+    errs, infos = M.opticalFlowMultiscale4dFused([v], [g0], [g1], **prm)
+    assert errs[0] == e and np.array_equal(v.to_numpy(), ref)
+    assert infos[0].summary()["levels"] == inf["levels"]
+
+
 def test_fused_batch_rejects_other_parameters():
     s = M.Solver(gridSize=M.vec3(8, 8, 8), fourthDim=8)
     g0, g1, v = s.create(M.Grid4Real), s.create(M.Grid4Real), s.create(M.Grid4Vec4)

@@ -100,3 +100,27 @@ def test_mesh_of_warped_frame():
     assert len(ref[2]) > 100
     for g, r in zip(got, ref):
         assert np.array_equal(g, r)
+
+
+def test_mesh_empty_and_smallest_grid(tmp_path):
+    """Edge cases: a level set with no surface gives an empty mesh (and a file
+    of three zero counts); the smallest grid (2 x 2 x 2: one cube) with one
+    corner inside gives the oracle's single triangle."""
+    phi = np.full((9, 8, 7), 3.0, np.float32)
+    mesh = M.Mesh()
+    M.DeviceArray.from_numpy(phi).createMesh(mesh)
+    assert mesh.numNodes() == 0 and mesh.numTris() == 0
+    ref = O.create_mesh(phi)
+    assert len(ref[0]) == 0 and len(ref[2]) == 0
+    path = tmp_path / "e.bobj.gz"
+    mesh.save(str(path))
+    assert gzip.open(path, "rb").read() == struct.pack("<iii", 0, 0, 0)
+    tiny = np.full((2, 2, 2), 1.0, np.float32)
+    tiny[0, 0, 0] = -0.6
+    mesh2 = M.Mesh()
+    M.DeviceArray.from_numpy(tiny).createMesh(mesh2)
+    ref2 = O.create_mesh(tiny)
+    say("one-cube mesh", len(ref2[0]), "nodes", len(ref2[2]), "triangles")
+    assert len(ref2[2]) == 1 and len(ref2[0]) == 3
+    for g, r in zip(mesh2.to_numpy(), ref2):
+        assert np.array_equal(g, r)

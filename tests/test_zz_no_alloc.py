@@ -57,6 +57,13 @@ def test_noalloc_ls_diff_matches_context_entry_twice():
     assert big > 0 and small > 0
 
 
+def test_noalloc_ls_diff_single_time_slice():
+    """A grid of one time slice: the 3D scaling of calcLsDiff4d (x1000 once).
+    (A border band applies to t as well, so with one slice only bnd 0
+    leaves cells to count.)"""
+    assert run_pair((1, 12, 13, 14), (5, 6), 0.02, 0) > 0
+
+
 def test_noalloc_ls_diff_rejects_bad_arguments():
     L = _lib.lib()
     res = M.DeviceArray((1,))
