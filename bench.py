@@ -356,6 +356,9 @@ def main():
         out["mixed_precision"] = mixed_precision(M)
         out["warm_start"] = warm_start_c5(M)
         out["batch"] = fused_batch(M, L)
+        # the same batch at sizes whose levels cannot fill the chip alone,
+        # where sharing launches pays
+        out["batch_small"] = fused_batch(M, L, ns=(16, 24, 32), ts=(16, 24, 32), min_grid=8)
         out["flof_params_32x48"] = flof_workload(M, cpu=False)
 
     if not args.no_cpu:
@@ -755,20 +758,23 @@ def warm_start_c5(M, n=128, t=64, seeds=(9001, 9002), nspheres=2):
     return out
 
 
-def fused_batch(M, L):
+def fused_batch(M, L, ns=(48, 64, 80), ts=(32, 48, 64), min_grid=None):
     """The 16-solve batch (forward and backward of 8 seeded drop pairs of
-    1-3 spheres, N in {48,64,80}, T in {32,48,64}): one solve after another
-    against the fused batch (every PCG launch spans the levels of equal index
-    of all problems); the results must be identical."""
+    1-3 spheres, N in ns, T in ts; by default {48,64,80} and {32,48,64}):
+    one solve after another against the fused batch (every PCG launch spans
+    the levels of equal index of all problems); the results must be
+    identical."""
     rng = np.random.default_rng(4)
     jobs = []
     for k in range(8):
-        n = int(rng.choice([48, 64, 80]))
-        t = int(rng.choice([32, 48, 64]))
+        n = int(rng.choice(list(ns)))
+        t = int(rng.choice(list(ts)))
         sv, a, b = _drop_pair(M, n, t, (100 + 2 * k, 101 + 2 * k), 1 + k % 3)
         for (x, y) in ((a, b), (b, a)):
             jobs.append((sv, x, y, sv.create(M.Grid4Vec4), (n, t)))
     prm = dict(PARAMS, cgMaxIter=FIXED_RESIDUAL_CAP)
+    if min_grid:
+        prm["minGridSize"] = min_grid
     # launch shapes per grid shape from the autotuner (both runs)
     L.flof_ctx_set_autotune(M.context(), 1)
     M.sync()
@@ -800,7 +806,7 @@ def fused_batch(M, L):
     log("bench: batch sequential %.2f s fused %.2f s identical %s" % (t_seq, t_f, same))
     cells = sum(j[4][0] ** 3 * j[4][1] for j in jobs)
     return {"workload": "16 solves: forward+backward of 8 seeded drop pairs (1-3 spheres), "
-                        "N in {48,64,80}, T in {32,48,64}, P_plain, residual 1e-5",
+                        "N in %s, T in %s, P_plain, residual 1e-5" % (list(ns), list(ts)),
             "sizes": [j[4] for j in jobs], "total_cells": cells,
             "sequential_s": t_seq, "fused_s": t_f, "fused_solves_per_s": len(jobs) / t_f,
             "speedup_vs_sequential": t_seq / t_f, "identical_to_sequential": same,
