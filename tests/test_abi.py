@@ -28,6 +28,36 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
 
 
+def test_header_is_plain_c_and_links(tmp_path):
+    """include/flof.h compiles as strict C99 (a C, Go/cgo or FFI caller
+    includes it as is) and a C program that takes the address of every
+    declared entry point links against libflof_hip.so."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    names = declared()
+    src = tmp_path / "uses_all.c"
+    src.write_text('#include "flof.h"\n#include <stdio.h>\n'
+                   "typedef void (*any_fn)(void);\n"
+                   "static const any_fn entry_points[] = {\n"
+                   + "".join("    (any_fn)&%s,\n" % n for n in names)
+                   + "};\nint main(void) {\n"
+                   '    printf("%d\\n", (int)(sizeof entry_points / sizeof entry_points[0]));\n'
+                   "    return 0;\n}\n")
+    inc = os.path.join(ROOT, "include")
+    r = subprocess.run([gcc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror",
+                        "-I", inc, "-c", str(src), "-o", str(tmp_path / "u.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    r = subprocess.run([gcc, str(tmp_path / "u.o"), "-o", str(tmp_path / "u"), "-L", libdir,
+                        "-lflof_hip", "-Wl,--unresolved-symbols=ignore-in-shared-libs"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_struct_layouts_match():
     L = _lib.lib()
     assert L.flof_abi_struct_size(0) == C.sizeof(OfParams)
