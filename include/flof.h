@@ -567,7 +567,10 @@ int flof_warmnet_get_params(flof_warmnet *net, float *host, int n);
 int flof_warmnet_set_params(flof_warmnet *net, const float *host, int n);
 /* Gradient of the last flof_warmnet_loss_grad call. */
 int flof_warmnet_get_grads(flof_warmnet *net, float *host, int n);
-/* flow (VEC4, cells of the input grid) = net(i0, i1). */
+/* flow (VEC4, cells of the input grid) = net(i0, i1).  The training entries
+ * (forward, loss_grad) keep every activation for the backward pass and take
+ * grids below 2^31/16 cells; the solver's own inference (the _net solve)
+ * takes larger ones. */
 int flof_warmnet_forward(flof_warmnet *net, flof_grid4d *i0, flof_grid4d *i1,
                          flof_grid4d *flow);
 /* Mean squared error against target (VEC4) and its gradient (kept in the
