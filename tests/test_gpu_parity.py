@@ -495,12 +495,14 @@ def test_stencil_kernels_bit_exact(kind):
     assert np.array_equal(out["lds", FLOF_PSTORE_BF16][0], out["tmarch", FLOF_PSTORE_BF16][0])
 
 
-@pytest.mark.parametrize("cap", [1, 2, 5, 6])
+@pytest.mark.parametrize("cap", [1, 2, 5, 6, 16, 17])
 def test_cg_cap_at_both_iteration_parities(cap):
     """The x update of an even PCG iteration is deferred to the next one: a
     solve that stops at the iteration cap after an even or an odd iteration
-    (caps 1, 5 / 2, 6) must still hold every x update, in the LDS-tiled and
-    the t-marching form, bit for bit as the oracle with the same cap."""
+    (caps 1, 5, 17 / 2, 6, 16; 16 and 17 end at and just after the host
+    loop's 16-iteration chunk) must still hold every x update, in the
+    LDS-tiled and the t-marching form, bit for bit as the oracle with the
+    same cap."""
     from ofblend_amd import _lib
     from ofblend_amd.abi import FLOF_STENCIL_AUTO, FLOF_STENCIL_LDS, FLOF_STENCIL_TMARCH
     X, Y, Z, T = 70, 9, 10, 11
