@@ -670,6 +670,10 @@ def ops_workload(M, s, i0, i1, vel, reps=10):
          lambda: M.gaussianBlur4d(tmp_v, sigma=1.0), 64 * cells,
          "two passes (into a temporary and back), each reading and writing "
          "16 B per cell"),
+        ("gaussianBlur4d Vec4 sigma 2 (625 taps)",
+         lambda: M.gaussianBlur4d(tmp_v, sigma=2.0), 64 * cells,
+         "two passes (into a temporary and back), each reading and writing "
+         "16 B per cell"),
         ("calcLsDiff4d", lambda: M.calcLsDiff4d(i0, i1), 8 * cells,
          "read 2 x 4 B per cell (one host round trip)"),
         ("corrVelsOf4d (vel restored before each call)", corr, None,

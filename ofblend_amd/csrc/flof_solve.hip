@@ -1420,10 +1420,12 @@ __global__ void k_of_finish(const float4 *__restrict__ x, float4 *__restrict__ o
 
 // ---------------------------------------------------------------------------
 // Passes over 3^4 neighbourhoods with LDS-staged planes.  A workgroup owns a
-// 64 x 2 x 2 column of cells for a segment of t and marches through it,
-// keeping the haloed planes t-1, t, t+1 (66 x 4 x 4 cells each) in a ring of
-// three LDS slots, so the 81 reads per cell hit LDS instead of the vector
-// caches.  Interior cells have all 81 neighbours in the grid.
+// TX x TY x 2 column of cells (64 x 2 x 2, or 32 x 4 x 2 on grids whose
+// width would leave 64-wide tiles largely empty) for a segment of t and
+// marches through it, keeping the haloed planes t-1, t, t+1 ((TX+2) x (TY+2)
+// x 4 cells each) in a ring of three LDS slots, so the 81 reads per cell hit
+// LDS instead of the vector caches.  Interior cells have all 81 neighbours in
+// the grid.
 //  MARCH_BOX81: knCvExpolBlur4d (optflow4d.cpp:613-626): unmarked interior
 //    cells take the 3^4 box mean, summed in the reference's loop order (t,
 //    z, y, x outermost to innermost, FP32, starting from zero); other cells
@@ -1432,23 +1434,31 @@ __global__ void k_of_finish(const float4 *__restrict__ x, float4 *__restrict__ o
 //    clipped for an interior cell), same order and operations; other cells
 //    are left unchanged.  `aux` is the weight table by squared distance.
 enum { MARCH_BOX81 = 0, MARCH_BLUR1 = 1 };
-constexpr int BX_X = 64, BX_Y = 2, BX_Z = 2;
-constexpr int BX_HX = BX_X + 2, BX_HY = BX_Y + 2, BX_HZ = BX_Z + 2;
-constexpr int BX_PLANE = BX_HX * BX_HY * BX_HZ;  // 1056 cells
-constexpr int BX_LOADS = (BX_PLANE + 255) / 256;  // per thread and plane
-static_assert(BX_X * BX_Y * BX_Z == 256, "one cell per thread");
+template <int TX>
+struct BoxTile {
+    static constexpr int X = TX, Y = TX == 64 ? 2 : 4, Z = 2;
+    static constexpr int HX = X + 2, HY = Y + 2, HZ = Z + 2;
+    static constexpr int PLANE = HX * HY * HZ;        // 1056 / 816 cells
+    static constexpr int LOADS = (PLANE + 255) / 256;  // per thread and plane
+    static_assert(X * Y * Z == 256, "one cell per thread");
+};
 
 struct BoxMarch {
+    int tx;                    // tile width: 64 or 32
     int ntx, nty, ntz, ntile;  // column tiles
     int seglen, nseg;          // t segments
 };
 
-template <class T, int OP>
+template <class T, int OP, int TX>
 __global__ void __launch_bounds__(256)
 k_march3_lds(const T *__restrict__ a, T *__restrict__ out, const float *__restrict__ aux,
              Dims4 D, BoxMarch B, float eps)
 {
     typedef VOps<T> O;
+    typedef BoxTile<TX> BT;
+    constexpr int BX_X = BT::X, BX_Y = BT::Y, BX_Z = BT::Z;
+    constexpr int BX_HX = BT::HX, BX_HY = BT::HY;
+    constexpr int BX_PLANE = BT::PLANE, BX_LOADS = BT::LOADS;
     __shared__ T pl[3][BX_PLANE];
     const int tile = blockIdx.x % B.ntile;
     const int seg = blockIdx.x / B.ntile;
@@ -1560,19 +1570,37 @@ k_march3_lds(const T *__restrict__ a, T *__restrict__ out, const float *__restri
     }
 }
 
-static BoxMarch make_boxmarch(flof_ctx *ctx, const Dims4 &D)
+// per_cu: workgroups per CU that the t segments aim at
+static BoxMarch make_boxmarch(flof_ctx *ctx, const Dims4 &D, int per_cu = 6)
 {
     BoxMarch B;
-    B.ntx = (D.X + BX_X - 1) / BX_X;
-    B.nty = (D.Y + BX_Y - 1) / BX_Y;
-    B.ntz = (D.Z + BX_Z - 1) / BX_Z;
+    // 32-wide tiles where they cover the x extent with 10% fewer lanes
+    const int w64 = (D.X + 63) / 64 * 64, w32 = (D.X + 31) / 32 * 32;
+    B.tx = 10 * w32 < 9 * w64 ? 32 : 64;
+    const int ty = B.tx == 64 ? BoxTile<64>::Y : BoxTile<32>::Y;
+    B.ntx = (D.X + B.tx - 1) / B.tx;
+    B.nty = (D.Y + ty - 1) / ty;
+    B.ntz = (D.Z + 1) / 2;
     B.ntile = B.ntx * B.nty * B.ntz;
-    const int target = ctx->num_cu * 6;  // three 50 KB workgroups per CU, twice over
+    const int target = ctx->num_cu * per_cu;  // 6: three 50 KB workgroups per CU, twice
     int nseg = (target + B.ntile - 1) / B.ntile;
     nseg = std::max(1, std::min(nseg, D.T));
     B.seglen = (D.T + nseg - 1) / nseg;
     B.nseg = (D.T + B.seglen - 1) / B.seglen;
     return B;
+}
+
+template <class T, int OP>
+static void launch_march3(flof_ctx *ctx, const BoxMarch &B, const T *src, T *dst,
+                          const float *aux, const Dims4 &D, float eps)
+{
+    const dim3 grid(B.ntile * B.nseg);
+    if (B.tx == 32)
+        hipLaunchKernelGGL((k_march3_lds<T, OP, 32>), grid, dim3(256), 0, ctx->stream, src, dst,
+                           aux, D, B, eps);
+    else
+        hipLaunchKernelGGL((k_march3_lds<T, OP, 64>), grid, dim3(256), 0, ctx->stream, src, dst,
+                           aux, D, B, eps);
 }
 
 // Gaussian blur (optflow4d.cpp:127-174): every interior cell sums the
@@ -1620,6 +1648,113 @@ k_blur4d(const T *__restrict__ src, T *__restrict__ dst, Dims4 D, Coord4 C, int 
     }
 }
 
+// The same blur for windows of s = 2 (5^4 taps) with LDS-staged planes.  A
+// workgroup owns a TX x TY x 2 column of cells of one component (the four
+// components of a Vec4 grid are four independent sums, one workgroup each)
+// and marches it through a t segment, keeping the planes t-2 .. t+2 with a
+// two-cell halo ((TX+4) x (TY+4) x 6 values) in a ring of five LDS slots, so
+// the 625 reads per cell hit LDS instead of the vector caches.  Window
+// positions outside the grid hold +0: the reference skips them, and adding
+// w * (+0) = +0 to a sum that started at +0 changes no bit (such a sum is
+// never -0).  The weight sum of a clipped window depends only on how far
+// the faces clip it: `wsum` holds it for each of the 4^4 clipping patterns
+// (per dimension the window of an interior cell is [-1, 2], [-2, 1], [-2, 2]
+// or [-1, 1]), summed on the host in the reference's order.
+struct Blur5W {
+    float w[17];  // weight by squared distance 0..16
+};
+__device__ __forceinline__ int blur5_pattern(int i, int n)
+{
+    return (min(2, i) - 1) * 2 + (min(2, n - 1 - i) - 1);
+}
+template <int TX>
+__global__ void __launch_bounds__(256)
+k_blur5_lds(const float *__restrict__ src, float *__restrict__ dst, int cstride, Dims4 D,
+            BoxMarch B, Blur5W W, const float *__restrict__ wsum, float eps)
+{
+    typedef BoxTile<TX> BT;
+    constexpr int HX = BT::X + 4, HY = BT::Y + 4, HZ = BT::Z + 4;
+    constexpr int PLANE = HX * HY * HZ;  // 2448 / 1728 values
+    constexpr int LOADS = (PLANE + 255) / 256;
+    __shared__ float pl[5][PLANE];
+    const int per = B.ntile * B.nseg;
+    const int comp = blockIdx.x / per;
+    const int rem = blockIdx.x - comp * per;
+    const int tile = rem % B.ntile, seg = rem / B.ntile;
+    const int bx = tile % B.ntx, by = (tile / B.ntx) % B.nty, bz = tile / (B.ntx * B.nty);
+    const int x0 = bx * BT::X, y0 = by * BT::Y, z0 = bz * BT::Z;
+    const int t0 = seg * B.seglen;
+    const int t1 = min(t0 + B.seglen, D.T);
+    const int tx = threadIdx.x % BT::X, ty = (threadIdx.x / BT::X) % BT::Y,
+              tz = threadIdx.x / (BT::X * BT::Y);
+    const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+    const bool own = gx < D.X && gy < D.Y && gz < D.Z;
+    src += comp;  // value of cell c at [c * cstride]
+    dst += comp;
+    // window cells this thread stages: e = threadIdx.x + 256 m
+    int64_t hoff[LOADS];  // cell offset inside a t-plane, -1: outside the grid
+#pragma unroll
+    for (int m = 0; m < LOADS; ++m) {
+        const int e = threadIdx.x + 256 * m;
+        hoff[m] = -1;
+        if (e < PLANE) {
+            const int hx = e % HX, hy = (e / HX) % HY, hz = e / (HX * HY);
+            const int x = x0 + hx - 2, y = y0 + hy - 2, z = z0 + hz - 2;
+            if (x >= 0 && x < D.X && y >= 0 && y < D.Y && z >= 0 && z < D.Z)
+                hoff[m] = x + (int64_t)D.X * y + D.sz * z;
+        }
+    }
+    float stv[LOADS];
+    auto fetch = [&](int p) {
+#pragma unroll
+        for (int m = 0; m < LOADS; ++m)
+            stv[m] = (p >= 0 && p < D.T && hoff[m] >= 0) ? src[(hoff[m] + D.st * p) * cstride]
+                                                         : 0.f;
+    };
+    auto stash = [&](int p) {
+        float *sl = pl[(p + 10) % 5];
+#pragma unroll
+        for (int m = 0; m < LOADS; ++m) {
+            const int e = threadIdx.x + 256 * m;
+            if (e < PLANE) sl[e] = stv[m];
+        }
+    };
+    // planes t0-2 .. t0+1 (those outside the grid hold +0)
+    for (int p = t0 - 2; p <= t0 + 1; ++p) {
+        fetch(p);
+        stash(p);
+    }
+    const int cl = (tx + 2) + HX * ((ty + 2) + HY * (tz + 2));
+    const bool own_in = own && gx >= 1 && gy >= 1 && gz >= 1 && gx < D.X - 1 &&
+                        gy < D.Y - 1 && gz < D.Z - 1;
+    const int pxyz = (blur5_pattern(gz, D.Z) * 4 + blur5_pattern(gy, D.Y)) * 4 +
+                     blur5_pattern(gx, D.X);
+    for (int t = t0; t < t1; ++t) {
+        fetch(t + 2);
+        __syncthreads();  // every wave is done with plane t-3's slot
+        stash(t + 2);
+        __syncthreads();
+        if (!own_in || t < 1 || t >= D.T - 1) continue;
+        // the reference's order: t, z, y, x from low to high
+        float val = 0.f;
+#pragma unroll
+        for (int vt = -2; vt <= 2; ++vt) {
+            const float *sl = pl[(t + vt + 10) % 5] + cl;
+#pragma unroll
+            for (int zk = -2; zk <= 2; ++zk)
+#pragma unroll
+                for (int yj = -2; yj <= 2; ++yj)
+#pragma unroll
+                    for (int xi = -2; xi <= 2; ++xi)
+                        val = val + sl[xi + HX * (yj + HY * zk)] *
+                                        W.w[vt * vt + zk * zk + yj * yj + xi * xi];
+        }
+        const float weight = wsum[blur5_pattern(t, D.T) * 64 + pxyz];
+        const int64_t c = gx + (int64_t)D.X * gy + D.sz * gz + D.st * t;
+        dst[c * cstride] = weight > eps ? val / weight : pl[t % 5][cl];
+    }
+}
+
 void gaussian_blur(flof_ctx *ctx, void *a, const Dims4 &D, bool vec4, float sigma,
                    int iter)
 {
@@ -1642,18 +1777,62 @@ void gaussian_blur(flof_ctx *ctx, void *a, const Dims4 &D, bool vec4, float sigm
     const int nb = grid_blocks_xcd(ctx, n);
     const float eps = 1e-6f;  // VECTOR_EPSILON of FP32 builds
     const BoxMarch BM = make_boxmarch(ctx, D);
+    // s = 2: the weight of each squared distance and the weight sums of the
+    // 4^4 clipping patterns (summed in the reference's order)
+    Blur5W w5{};
+    Tmp wsum5;
+    BoxMarch BM5{};
+    if (s == 2) {
+        for (int d = 0; d < 17; ++d) w5.w[d] = wh[d];
+        std::vector<float> ws(256);
+        for (int pt = 0; pt < 4; ++pt)
+            for (int pz = 0; pz < 4; ++pz)
+                for (int py = 0; py < 4; ++py)
+                    for (int px = 0; px < 4; ++px) {
+                        float weight = 0.f;
+                        const int pat[4] = {pt, pz, py, px};
+                        int lo[4], hi[4];
+                        for (int q = 0; q < 4; ++q) {
+                            lo[q] = -(pat[q] / 2 + 1);
+                            hi[q] = pat[q] % 2 + 1;
+                        }
+                        for (int vt = lo[0]; vt <= hi[0]; ++vt)
+                            for (int zk = lo[1]; zk <= hi[1]; ++zk)
+                                for (int yj = lo[2]; yj <= hi[2]; ++yj)
+                                    for (int xi = lo[3]; xi <= hi[3]; ++xi)
+                                        weight = weight +
+                                                 wh[vt * vt + zk * zk + yj * yj + xi * xi];
+                        ws[((pt * 4 + pz) * 4 + py) * 4 + px] = weight;
+                    }
+        wsum5 = Tmp(ctx, 256 * sizeof(float));
+        FLOF_HIP(hipMemcpyAsync(wsum5.p, ws.data(), 256 * sizeof(float),
+                                hipMemcpyHostToDevice, ctx->stream));
+        FLOF_HIP(hipStreamSynchronize(ctx->stream));  // ws lives on the stack
+        BM5 = make_boxmarch(ctx, D, vec4 ? 2 : 6);
+    }
     auto pass = [&](const void *src, void *dst) {
+        if (s == 2) {
+            const int ncomp = vec4 ? 4 : 1;
+            const dim3 grid(BM5.ntile * BM5.nseg * ncomp);
+            if (BM5.tx == 32)
+                hipLaunchKernelGGL(k_blur5_lds<32>, grid, dim3(256), 0, ctx->stream,
+                                   (const float *)src, (float *)dst, ncomp, D, BM5, w5,
+                                   (const float *)wsum5.as<float>(), eps);
+            else
+                hipLaunchKernelGGL(k_blur5_lds<64>, grid, dim3(256), 0, ctx->stream,
+                                   (const float *)src, (float *)dst, ncomp, D, BM5, w5,
+                                   (const float *)wsum5.as<float>(), eps);
+            check_launch("k_blur5_lds");
+            return;
+        }
         if (s == 1) {
             // the LDS-tiled form of the same sums (identical results)
-            const dim3 grid(BM.ntile * BM.nseg);
             if (vec4)
-                hipLaunchKernelGGL((k_march3_lds<float4, MARCH_BLUR1>), grid, dim3(256), 0,
-                                   ctx->stream, (const float4 *)src, (float4 *)dst,
-                                   (const float *)wdev.as<float>(), D, BM, eps);
+                launch_march3<float4, MARCH_BLUR1>(ctx, BM, (const float4 *)src, (float4 *)dst,
+                                                   (const float *)wdev.as<float>(), D, eps);
             else
-                hipLaunchKernelGGL((k_march3_lds<float, MARCH_BLUR1>), grid, dim3(256), 0,
-                                   ctx->stream, (const float *)src, (float *)dst,
-                                   (const float *)wdev.as<float>(), D, BM, eps);
+                launch_march3<float, MARCH_BLUR1>(ctx, BM, (const float *)src, (float *)dst,
+                                                  (const float *)wdev.as<float>(), D, eps);
             check_launch("k_march3_lds<blur1>");
             return;
         }
@@ -1787,9 +1966,8 @@ void corr_vels_of(flof_ctx *ctx, float4 *dst, float4 *vel, const float *phiOrg,
         float4 *cur = dst, *nxt = tmp.as<float4>();
         const BoxMarch BM = make_boxmarch(ctx, D);
         for (int it = 0; it < doCvExtrapol; ++it) {
-            hipLaunchKernelGGL((k_march3_lds<float4, MARCH_BOX81>), dim3(BM.ntile * BM.nseg),
-                               dim3(256), 0, ctx->stream, (const float4 *)cur, nxt,
-                               (const float *)marker.as<float>(), D, BM, 0.f);
+            launch_march3<float4, MARCH_BOX81>(ctx, BM, (const float4 *)cur, nxt,
+                                               (const float *)marker.as<float>(), D, 0.f);
             check_launch("k_march3_lds<box81>");
             std::swap(cur, nxt);
         }

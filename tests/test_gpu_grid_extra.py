@@ -199,27 +199,55 @@ def test_reductions_mult_and_vec4_region():
 
 
 @pytest.mark.parametrize("vec", [True, False])
-def test_gaussian_blur_two_x_tiles_bit_exact(vec):
-    """gaussianBlur4d at sigma 1 (the LDS column march over 64-wide x tiles)
-    on a grid 70 wide: the seam between the two x tiles and the ragged second
-    tile equal the oracle bit for bit, Vec4 and Real."""
-    X, Y, Z, T = 70, 9, 10, 11
+@pytest.mark.parametrize("X", [40, 70, 100])
+@pytest.mark.parametrize("sigma", [1.0, 2.0])
+def test_gaussian_blur_two_x_tiles_bit_exact(vec, X, sigma):
+    """gaussianBlur4d at sigma 1 and 2 (the LDS column marches over x tiles
+    with 3^4 and 5^4 windows) on grids 40, 70 and 100 wide (one ragged
+    64-wide tile, three 32-wide tiles, two 64-wide tiles): the seams between
+    the x tiles and the ragged last tile equal the oracle bit for bit, Vec4
+    and Real."""
+    Y, Z, T = 9, 10, 11
     rng = np.random.default_rng(12)
     shp = (T, Z, Y, X, 4) if vec else (T, Z, Y, X)
     a = rng.standard_normal(shp).astype(np.float32)
     s = M.Solver(gridSize=M.vec3(X, Y, Z), fourthDim=T)
     g = s.create(M.Grid4Vec4 if vec else M.Grid4Real)
     g.from_numpy(a)
-    say("gaussian blur 70x9x10x11 vec", vec)
-    M.gaussianBlur4d(g, 1.0, 1)
-    assert np.array_equal(g.to_numpy(), O.gaussian_blur(a, 1.0, 1))
+    say("gaussian blur", (X, Y, Z, T), "sigma", sigma, "vec", vec)
+    M.gaussianBlur4d(g, sigma, 1)
+    assert np.array_equal(g.to_numpy(), O.gaussian_blur(a, sigma, 1))
+
+
+@pytest.mark.parametrize("shape", [(3, 6, 4, 5), (7, 3, 5, 3), (5, 5, 5, 6)])
+def test_gaussian_blur_5x5_window_small_dims_bit_exact(shape):
+    """The 5^4 window (sigma 2) on grids with 3 to 7 cells per dimension: the
+    windows clipped on one side, on both sides ([-1, 1] where a dimension has
+    3 cells) and the weight sums of every such clipping pattern equal the
+    oracle bit for bit, Vec4 (two iterations) and Real."""
+    X, Y, Z, T = shape
+    rng = np.random.default_rng(14)
+    s = M.Solver(gridSize=M.vec3(X, Y, Z), fourthDim=T)
+    say("gaussian blur 5^4 window", shape)
+    a4 = rng.standard_normal((T, Z, Y, X, 4)).astype(np.float32)
+    g4 = s.create(M.Grid4Vec4)
+    g4.from_numpy(a4)
+    M.gaussianBlur4d(g4, 2.0, 2)
+    assert np.array_equal(g4.to_numpy(), O.gaussian_blur(a4, 2.0, 2))
+    a1 = rng.standard_normal((T, Z, Y, X)).astype(np.float32)
+    g1 = s.create(M.Grid4Real)
+    g1.from_numpy(a1)
+    M.gaussianBlur4d(g1, 1.7, 1)
+    assert np.array_equal(g1.to_numpy(), O.gaussian_blur(a1, 1.7, 1))
 
 
 @pytest.mark.parametrize("vx", [0.0, 1.0])
-def test_corr_vels_of4d_two_x_tiles_bit_exact(vx):
-    """corrVelsOf4d on a grid 70 wide (two x tiles of the box passes): 4 and
-    5 extrapolation passes, random level sets, bit-exact to the oracle."""
-    X, Y, Z, T = 70, 10, 9, 8
+@pytest.mark.parametrize("X", [40, 70, 100])
+def test_corr_vels_of4d_two_x_tiles_bit_exact(vx, X):
+    """corrVelsOf4d on grids 40, 70 and 100 wide (the box passes on one
+    ragged 64-wide x tile, three 32-wide tiles, two 64-wide tiles): 4 and 5
+    extrapolation passes, random level sets, bit-exact to the oracle."""
+    Y, Z, T = 10, 9, 8
     rng = np.random.default_rng(13)
     t, z, y, x = np.meshgrid(*[np.arange(s) + 0.5 for s in (T, Z, Y, X)], indexing="ij")
     base = x - 0.45 * X + 0.3 * np.sin(0.7 * y) + 0.2 * np.cos(0.5 * z + 0.3 * t)
@@ -233,7 +261,7 @@ def test_corr_vels_of4d_two_x_tiles_bit_exact(vx):
     po, pt = s.create(M.Grid4Real), s.create(M.Grid4Real)
     po.from_numpy(phiOrg)
     pt.from_numpy(phiTgt)
-    say("corrVelsOf4d 70 wide vx", vx)
+    say("corrVelsOf4d", (X, Y, Z, T), "vx", vx)
     M.corrVelsOf4d(dst, vel, po, po, pt, threshPhi=4.0, postVelBlur=0.0, maxIter=40)
     rd, rv = O.corr_vels_of4d(vel0, phiOrg, phiTgt, threshPhi=4.0, postVelBlur=0.0,
                               maxIter=40)
