@@ -1885,6 +1885,25 @@ __global__ void k_project_cells(float4 *__restrict__ dst, const float4 *__restri
         const float4 v = f4muls(vel[c], dt);
         const float px = ((float)i + 0.5f) - v.x, py = ((float)j + 0.5f) - v.y,
                     pz = ((float)k + 0.5f) - v.z, pt = ((float)t + 0.5f) - v.w;
+        // line search along the normal (projectCell, optflow4d.cpp:675-710);
+        // a cell whose final |d| exceeds threshPhi gets dst = 0 whatever its
+        // normal and path, and every remaining step moves d by at most the
+        // current step (which never grows), so a cell stops as soon as no
+        // remaining steps can bring |d| back within threshPhi (with a margin
+        // far above the FP32 rounding of the steps); the result is the same
+        float step = 0.25f;
+        if (maxIter < 50) step = 0.5f;
+        const float target = phiTarget[c];
+        float vn = interp4<float>(phiOrg, D, px, py, pz, pt);
+        float d = vn - target;
+        auto out_of_reach = [&](int rem) {
+            const float ad = fabsf(d);
+            return ad > threshPhi + (float)max(rem, 0) * step * 1.001f + 1e-3f * (1.f + ad);
+        };
+        if (out_of_reach(maxIter)) {
+            dst[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
+        }
         const float h = 0.5f;
         float4 nrm;
         nrm.x = interp4<float>(phiOrg, D, px + h, py, pz, pt) -
@@ -1906,13 +1925,7 @@ __global__ void k_project_cells(float4 *__restrict__ dst, const float4 *__restri
                 nrm = make_float4(0.f, 0.f, 0.f, 0.f);
             }
         }
-        // line search along the normal (projectCell, optflow4d.cpp:675-710)
-        float step = 0.25f;
-        if (maxIter < 50) step = 0.5f;
         int lastDir = 0;
-        const float target = phiTarget[c];
-        float vn = interp4<float>(phiOrg, D, px, py, pz, pt);
-        float d = vn - target;
         for (int s = 0; s < maxIter; ++s) {
             const float4 o = f4muls(f4muls(nrm, d), dt);
             const float qx = px + o.x, qy = py + o.y, qz = pz + o.z, qt = pt + o.w;
@@ -1931,7 +1944,12 @@ __global__ void k_project_cells(float4 *__restrict__ dst, const float4 *__restri
                 d -= step;
                 lastDir = -1;
             }
-            if (step < 1e-01) s = maxIter + 1;
+            if (step < 1e-01) {
+                s = maxIter + 1;
+            } else if (out_of_reach(maxIter - 1 - s)) {
+                d = 9999.f;
+                s = maxIter + 1;
+            }
         }
         if (fabsf(d) > threshPhi) {
             dst[c] = make_float4(0.f, 0.f, 0.f, 0.f);

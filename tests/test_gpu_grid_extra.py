@@ -81,6 +81,37 @@ def test_advect_cfl_bit_exact(cfl):
     assert np.array_equal(gg.to_numpy(), ref)
 
 
+@pytest.mark.parametrize("thresh,maxIter", [(4.0, 40), (1.5, 60), (4.0, 0), (1.0, 7),
+                                            (0.3, 45)])
+def test_corr_vels_of4d_far_cells_bit_exact(thresh, maxIter):
+    """The projection's line search on level sets whose offsets span 0 to
+    about 20 cells: cells that start out of reach of threshPhi, cells whose
+    search runs away, cells that end just inside or outside threshPhi, both
+    initial steps (maxIter below and above 50) and no search at all
+    (maxIter 0): the field and vel equal the oracle's bit for bit."""
+    X, Y, Z, T = 24, 11, 10, 9
+    rng = np.random.default_rng(21)
+    t, z, y, x = np.meshgrid(*[np.arange(n) + 0.5 for n in (T, Z, Y, X)], indexing="ij")
+    base = np.sqrt((x - 9.0) ** 2 + (y - 5.0) ** 2 + (z - 4.5) ** 2) - 3.0 + 0.2 * t
+    phiOrg = (base + 0.05 * rng.standard_normal(base.shape)).astype(np.float32)
+    phiTgt = (2.5 * base - 2.3 + 0.4 * np.sin(0.9 * x) + 0.05 *
+              rng.standard_normal(base.shape)).astype(np.float32)
+    vel0 = (rng.standard_normal((T, Z, Y, X, 4)) * 0.7).astype(np.float32)
+    s = M.Solver(gridSize=M.vec3(X, Y, Z), fourthDim=T)
+    dst, vel = s.create(M.Grid4Vec4), s.create(M.Grid4Vec4)
+    vel.from_numpy(vel0)
+    po, pt = s.create(M.Grid4Real), s.create(M.Grid4Real)
+    po.from_numpy(phiOrg)
+    pt.from_numpy(phiTgt)
+    say("corrVelsOf4d far cells", (X, Y, Z, T), "thresh", thresh, "maxIter", maxIter)
+    M.corrVelsOf4d(dst, vel, po, po, pt, threshPhi=thresh, postVelBlur=0.0, maxIter=maxIter)
+    rd, rv = O.corr_vels_of4d(vel0, phiOrg, phiTgt, threshPhi=thresh, postVelBlur=0.0,
+                              maxIter=maxIter)
+    assert np.abs(rd).max() > 0.0
+    assert np.array_equal(dst.to_numpy(), rd)
+    assert np.array_equal(vel.to_numpy(), rv)
+
+
 @pytest.mark.parametrize("vx", [0.0, 1.0])
 def test_corr_vels_of4d_planar_and_oracle(vx):
     """corrVelsOf4d on its own: planar level sets offset by a known distance
