@@ -1608,7 +1608,9 @@ static void launch_march3(flof_ctx *ctx, const BoxMarch &B, const T *src, T *dst
 // from low to high), the weights and the values in FP32, then divides by
 // the weight sum.  The weight of a squared distance d is
 // (float)exp(-d / (2 sigma^2)) evaluated in double on the host, one table
-// entry per d in [0, 4 s^2], so the device uses the host's exp.
+// entry per d in [0, 4 s^2], so the device uses the host's exp.  This
+// gathering form serves windows of s >= 3; s = 1 and s = 2 run as the LDS
+// marches k_march3_lds and k_blur5_lds with the same sums.
 template <class T>
 __global__ void __launch_bounds__(256)
 k_blur4d(const T *__restrict__ src, T *__restrict__ dst, Dims4 D, Coord4 C, int s,
