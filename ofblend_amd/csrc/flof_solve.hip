@@ -1582,7 +1582,9 @@ static BoxMarch make_boxmarch(flof_ctx *ctx, const Dims4 &D, int per_cu = 6)
     B.nty = (D.Y + ty - 1) / ty;
     B.ntz = (D.Z + 1) / 2;
     B.ntile = B.ntx * B.nty * B.ntz;
-    const int target = ctx->num_cu * per_cu;  // 6: three 50 KB workgroups per CU, twice
+    // the default 6: twice the workgroups per CU that LDS admits at once for
+    // 64-wide Vec4 tiles (three of 50 KB)
+    const int target = ctx->num_cu * per_cu;
     int nseg = (target + B.ntile - 1) / B.ntile;
     nseg = std::max(1, std::min(nseg, D.T));
     B.seglen = (D.T + nseg - 1) / nseg;
